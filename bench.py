@@ -1,0 +1,329 @@
+"""Benchmark of the MI355X Goofy encoder; prints ONE JSON line.
+
+Headline (`value`): plain encode of one 8192x8192 RGBA8 image (seeded waves +
+noise 12) already resident in HBM; one step = the DXT1 encode plus the ETC1S
+encode of that image (one kernel launch each), so MP/s counts the pixels of
+both formats per second of wall time over the timed steps.  Four input
+buffers of 256 MiB are rotated so that no launch reads an image the 256 MB
+last-level cache still holds.
+
+Also reported: per-format kernel times (HIP events on the launch stream) and
+the HBM roofline of the slower kernel; the batched workload (4096 textures,
+one persistent graph launch vs 4096 plain launches); the serving workload
+(Poisson arrivals, p50/p99 latency); and the reference encoder on the host
+CPU (compiled from its own source, one stripe of block rows per thread).
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MP/s bit-exact DXT1/ETC1S (batched+served vs plain launch); p50/p99 latency"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def aligned_dev(torch, nbytes: int, align: int = 256):
+    raw = torch.empty(nbytes + align, dtype=torch.uint8, device="cuda")
+    off = (-raw.data_ptr()) % align
+    return raw[off:off + nbytes]
+
+
+# ---- plain 8192^2 ------------------------------------------------------------------------
+
+def bench_plain(g, torch, W, steps: int, warmup: int, rotate: int, cfg: dict | None):
+    im = W.IMAGE_8192
+    nbytes = im.pitch * im.height
+    out_bytes = im.blocks * 8
+    log(f"plain: generating {rotate} x {im.name} on device")
+    srcs = []
+    for _ in range(rotate):
+        s = aligned_dev(torch, nbytes)
+        g.synth_device(im.kind, im.seed, im.texture_id, im.noise, s, im.width, im.height, im.pitch)
+        srcs.append(s)
+    outs = {f: [aligned_dev(torch, out_bytes) for _ in range(rotate)] for f in (W.DXT1, W.ETC1)}
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+
+    # parity of the measured path against the reference's digests
+    digests = json.loads((ROOT / "tests" / "golden" / "digests.json").read_text())["images"][im.name]
+    parity = {}
+    for f, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+        g.compress_device(f, outs[f][0], srcs[0], im.width, im.height, im.pitch, stream, cfg)
+        torch.cuda.synchronize()
+        parity[key] = sha(outs[f][0].cpu().numpy()) == digests[f"{key}_sha256"]
+    parity["input"] = sha(srcs[0].cpu().numpy()) == digests["input_sha256"]
+
+    j = 0
+
+    def step(events=None):
+        nonlocal j
+        for f in (W.DXT1, W.ETC1):
+            k = j % rotate
+            if events is not None:
+                events[f].append((torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)))
+                events[f][-1][0].record(stream)
+            g.compress_device(f, outs[f][k], srcs[k], im.width, im.height, im.pitch, stream, cfg)
+            if events is not None:
+                events[f][-1][1].record(stream)
+            j += 1
+
+    log(f"plain: warmup {warmup} steps")
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    # timed region: wall clock around exactly `steps` steps, events per launch
+    events = {W.DXT1: [], W.ETC1: []}
+    log(f"plain: timing {steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(events)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern_ms = {f: float(np.mean([a.elapsed_time(b) for a, b in events[f]])) for f in events}
+    kern_med = {f: float(np.median([a.elapsed_time(b) for a, b in events[f]])) for f in events}
+    del srcs, outs
+    torch.cuda.empty_cache()
+    return im, elapsed, kern_ms, kern_med, parity
+
+
+# ---- batched --------------------------------------------------------------------------------
+
+def bench_batch(g, torch, W, reps: int = 7):
+    ts = W.batch_textures()
+    sizes = [t["width"] * 4 * t["height"] for t in ts]
+    offs = np.concatenate([[0], np.cumsum([(s + 255) // 256 * 256 for s in sizes])])
+    osz = [(t["width"] // 4) * (t["height"] // 4) * 8 for t in ts]
+    ooffs = np.concatenate([[0], np.cumsum([(s + 255) // 256 * 256 for s in osz])])
+    log(f"batch: {len(ts)} textures, {offs[-1] / 2**30:.2f} GiB in, generating on device")
+    src = aligned_dev(torch, int(offs[-1]))
+    dst = aligned_dev(torch, int(ooffs[-1]))
+    dst2 = aligned_dev(torch, int(ooffs[-1]))
+    base, obase, obase2 = src.data_ptr(), dst.data_ptr(), dst2.data_ptr()
+    items, items2 = [], []
+    for i, t in enumerate(ts):
+        g.synth_device(t["kind"], t["seed"], t["texture_id"], t["noise"], base + int(offs[i]), t["width"], t["height"])
+        d = dict(src=base + int(offs[i]), width=t["width"], height=t["height"], pitch=t["width"] * 4,
+                 format=t["format"])
+        items.append({**d, "dst": obase + int(ooffs[i])})
+        items2.append({**d, "dst": obase2 + int(ooffs[i])})
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    b = g.Batch(items)
+    b2 = g.Batch(items2)
+    info = b.info()
+    log("batch: autotuning")
+    chosen, tuned_ms = b.autotune(stream, reps=2)
+
+    def timed(fn, n):
+        ts_ = []
+        for _ in range(n):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn(stream)
+            e1.record(stream)
+            e1.synchronize()
+            ts_.append(e0.elapsed_time(e1))
+        return float(np.median(ts_))
+
+    b.launch(stream)
+    torch.cuda.synchronize()
+    graph_ms = timed(b.launch, reps)
+    b2.launch_per_texture(stream)
+    torch.cuda.synchronize()
+    plain_ms = timed(b2.launch_per_texture, max(3, reps // 2))
+    torch.cuda.synchronize()
+    same = bool(torch.equal(dst, dst2))
+    px = info["pixels"]
+    res = {
+        "workload": "4096 textures, W,H log-uniform multiples of 4 in [64,2048], 50/50 DXT1/ETC1S, noise U[0,40]",
+        "textures": info["textures"], "megapixels": round(px / 1e6, 2),
+        "graph_ms": round(graph_ms, 4), "mps": round(px / 1e6 / (graph_ms / 1e3), 1),
+        "per_texture_launch_ms": round(plain_ms, 4), "per_texture_mps": round(px / 1e6 / (plain_ms / 1e3), 1),
+        "speedup_vs_per_texture": round(plain_ms / graph_ms, 2),
+        "gbps": round(px / 16 * 72 / (graph_ms / 1e3) / 1e9, 1),
+        "autotuned_config": chosen, "bit_exact_vs_per_texture": same,
+    }
+    b.close()
+    b2.close()
+    del src, dst, dst2
+    torch.cuda.empty_cache()
+    return res
+
+
+# ---- CPU baseline ------------------------------------------------------------------------------
+
+def cpu_baseline(g, W, budget_s: float = 3.0):
+    from oracle.oracle import Oracle, Reference, aligned_empty, have_reference
+    threads = max(1, min(16, os.cpu_count() or 1))
+    im = W.IMAGE_8192
+    img = g.synth_host(im.kind, im.seed, im.texture_id, im.noise, im.width, im.height, im.pitch)
+    kind = "reference" if have_reference() else "port"
+    if kind == "reference":
+        ref = Reference()
+        outs = {}
+
+        def enc(f, img_, w, h, p, t):
+            if w not in outs:
+                outs[w] = aligned_empty((w // 4) * (h // 4) * 8)
+            ref.compress_mt(f, img_, w, h, p, t, out=outs[w])
+    else:
+        ora = Oracle()
+
+        def enc(f, img_, w, h, p, t):
+            ora.compress_blocks(f, img_, w, h, p, threads=t)
+    times = {0: [], 1: []}
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < budget_s or min(len(v) for v in times.values()) < 2:
+        for f in (0, 1):
+            t0 = time.perf_counter()
+            enc(f, img, im.width, im.height, im.pitch, threads)
+            times[f].append(time.perf_counter() - t0)
+        if min(len(v) for v in times.values()) >= 64:
+            break
+    best = {f: min(v) for f, v in times.items()}
+    reps = len(times[0])
+    # the reference's own single-thread case: 512^2, best of 128 (Src/main.cpp:43,653-664)
+    s = W.IMAGE_512
+    small = g.synth_host(s.kind, s.seed, s.texture_id, s.noise, s.width, s.height, s.pitch)
+    st = {}
+    for f in (0, 1):
+        bt = 1e9
+        for _ in range(128):
+            t0 = time.perf_counter()
+            enc(f, small, s.width, s.height, s.pitch, 1)
+            bt = min(bt, time.perf_counter() - t0)
+        st["dxt1" if f == 0 else "etc1"] = round(s.pixels / 1e6 / bt, 1)
+    mps = 2 * im.pixels / 1e6 / (best[0] + best[1])
+    return {
+        "value": round(mps, 1), "unit": "MP/s", "cores": threads, "kind": kind,
+        "sample": f"{im.name} DXT1+ETC1S, {reps} encodes per format, best of, one stripe of block rows per thread",
+        "per_format_mps": {"dxt1": round(im.pixels / 1e6 / best[0], 1), "etc1": round(im.pixels / 1e6 / best[1], 1)},
+        "single_thread_512_mps": st,
+    }
+
+
+def load_traffic():
+    p = ROOT / "profiles" / "hbm_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text())
+    except Exception:
+        return None
+
+
+def profile_mode(g, torch, W, launches: int):
+    """Short run for rocprofv3: each plain kernel `launches` times, plus a
+    plain 256 MiB int32 copy kernel used to calibrate the HBM byte counters."""
+    im = W.IMAGE_8192
+    src = aligned_dev(torch, im.pitch * im.height)
+    g.synth_device(im.kind, im.seed, im.texture_id, im.noise, src, im.width, im.height, im.pitch)
+    out = aligned_dev(torch, im.blocks * 8)
+    a = src.view(torch.int32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for i in range(launches):
+        torch.bitwise_or(a, 0, out=b)  # calibration: reads and writes 256 MiB
+    for f in (W.DXT1, W.ETC1):
+        for i in range(launches):
+            g.compress_device(f, out, src, im.width, im.height, im.pitch)
+    torch.cuda.synchronize()
+    log(f"profile mode done: {launches} launches per kernel")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--rotate", type=int, default=4)
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--profile", type=int, default=0, help="rocprof mode: N launches per kernel, no JSON")
+    ap.add_argument("--bpl", type=int, default=0)
+    ap.add_argument("--wg", type=int, default=0)
+    ap.add_argument("--nt", type=int, default=-1)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the encoder needs no collective, and the multi-GPU "
+              "configuration is not part of this build (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import goofy_amd as g
+    from goofy_amd import workloads as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    if args.profile:
+        profile_mode(g, torch, W, args.profile)
+        return 0
+
+    cfg = None
+    if args.bpl or args.wg or args.nt >= 0:
+        cfg = dict(blocks_per_lane=args.bpl or 2, wg_waves=args.wg or 4, waves_per_cu=0,
+                   nontemporal=max(0, args.nt))
+
+    im, elapsed, kern_ms, kern_med, parity = bench_plain(g, torch, W, args.steps, args.warmup, args.rotate, cfg)
+    ms_per_step = elapsed / args.steps * 1e3
+    value = 2 * im.pixels / 1e6 / (ms_per_step / 1e3)
+    algo = W.algorithmic_bytes(im.width, im.height)
+    dom = max(kern_ms, key=kern_ms.get)
+    achieved = algo / (kern_ms[dom] / 1e3) / 1e9
+    traffic = load_traffic()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": (traffic or {}).get("bytes_per_launch", {}).get("dxt1" if dom == W.DXT1 else "etc1"),
+                "kernel": "encode ETC1S" if dom == W.ETC1 else "encode DXT1",
+                "algorithmic_bytes_per_launch": algo}
+    formats = {}
+    for f, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+        formats[key] = {"kernel_ms_mean": round(kern_ms[f], 5), "kernel_ms_median": round(kern_med[f], 5),
+                        "kernel_mps": round(im.pixels / 1e6 / (kern_ms[f] / 1e3), 1),
+                        "kernel_gbps": round(algo / (kern_ms[f] / 1e3) / 1e9, 1)}
+    result = {
+        "metric": METRIC, "value": round(value, 1), "unit": "MP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"plain encode, {im.name} RGBA8 stride {im.pitch} B, waves+noise12 seed "
+                               f"{im.seed:#x}; step = DXT1 + ETC1S launch, inputs HBM-resident, 4-buffer rotation",
+                   "global_batch": 1, "formats": ["DXT1", "ETC1S"], "parallelism": "single GPU"},
+        "roofline": roofline, "formats": formats, "parity": parity,
+    }
+    if not args.skip_batch:
+        result["batched"] = bench_batch(g, torch, W)
+    if not args.skip_serve and hasattr(g, "serve_simulate"):
+        from goofy_amd import serving
+        result["served"] = serving.bench_serving(g, torch, W, log=log)
+    if not args.skip_cpu:
+        log("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(g, W)
+    if traffic:
+        result["roofline"]["traffic_source"] = traffic.get("source")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,268 @@
+"""goofy_amd -- Python host mirror of the Goofy encoder API on MI355X.
+
+Everything here is a thin ctypes layer over the C-ABI of ``libgoofy_hip.so``
+(declared in ``include/goofy_hip.h``); all encoding runs in the HIP kernels of
+that library.  There is no CPU fallback: importing this package without the
+built library raises ImportError.
+
+Reference interfaces mirrored:
+  * ``compressDXT1`` / ``compressETC1`` -- ``goofy::compressDXT1/ETC1``
+    (GoofyTC/goofy_tc.h:10-13, :1497-1557): same argument meaning, same
+    return codes (-1 width % 16, -2 height % 4), host buffers.
+  * ``CompressFunc`` -- the harness plugin type ``CompressFunc_t``
+    (Src/main.cpp:644): the two C entry points with exactly that signature.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+from typing import Iterable, Sequence
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libgoofy_hip.so"
+
+DXT1 = 0
+ETC1 = 1
+FORMATS = {"dxt1": DXT1, "etc1": ETC1, "etc1s": ETC1}
+
+OK = 0
+ERR_WIDTH = -1
+ERR_HEIGHT = -2
+ERR_HIP = -3
+ERR_INVALID = -4
+ERR_OUT_OF_MEMORY = -5
+ERR_NO_DEVICE = -6
+ERR_TOO_LARGE = -7
+ERR_BUSY = -8
+
+
+class GoofyError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"goofy status {status}: {message}")
+        self.status = status
+
+
+if not _LIB_PATH.exists():
+    raise ImportError(f"{_LIB_PATH} is not built: run `python -m goofy_amd.build` "
+                      "(the encoder has no CPU fallback)")
+
+_lib = ctypes.CDLL(str(_LIB_PATH))
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+_uint = ctypes.c_uint
+_sz = ctypes.c_size_t
+
+CompressFunc = ctypes.CFUNCTYPE(ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
+
+
+class LaunchConfig(ctypes.Structure):
+    _fields_ = [("blocks_per_lane", ctypes.c_int), ("wg_waves", ctypes.c_int),
+                ("waves_per_cu", ctypes.c_int), ("nontemporal", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class TextureDesc(ctypes.Structure):
+    _fields_ = [("src", _vp), ("dst", _vp), ("width", _uint), ("height", _uint),
+                ("pitch", _sz), ("format", ctypes.c_int)]
+
+
+def _sig(name, restype, *argtypes):
+    fn = getattr(_lib, name)
+    fn.restype = restype
+    fn.argtypes = list(argtypes)
+    return fn
+
+
+_sig("goofy_hip_last_error", ctypes.c_char_p)
+_sig("goofy_hip_status_string", ctypes.c_char_p, ctypes.c_int)
+_sig("goofy_hip_version", ctypes.c_char_p)
+_sig("goofy_hip_device_count", ctypes.c_int)
+_sig("goofy_hip_compress", ctypes.c_int, ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
+_sig("goofy_hip_compress_dxt1", ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
+_sig("goofy_hip_compress_etc1", ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
+_sig("goofy_hip_compress_device", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp)
+_sig("goofy_hip_compress_device_cfg", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp,
+     ctypes.POINTER(LaunchConfig))
+_sig("goofy_batch_create", ctypes.c_int, ctypes.POINTER(TextureDesc), _sz, ctypes.POINTER(LaunchConfig),
+     ctypes.POINTER(_vp))
+_sig("goofy_batch_launch", ctypes.c_int, _vp, _vp)
+_sig("goofy_batch_launch_per_texture", ctypes.c_int, _vp, _vp)
+_sig("goofy_batch_autotune", ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.POINTER(LaunchConfig),
+     ctypes.POINTER(ctypes.c_float))
+_sig("goofy_batch_get_config", ctypes.c_int, _vp, ctypes.POINTER(LaunchConfig))
+_sig("goofy_batch_set_config", ctypes.c_int, _vp, ctypes.POINTER(LaunchConfig))
+_sig("goofy_batch_info", ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+     ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64))
+_sig("goofy_batch_destroy", ctypes.c_int, _vp)
+_sig("goofy_synth_fill_host", ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _u8p,
+     _uint, _uint, _sz, ctypes.c_int)
+_sig("goofy_synth_fill_device", ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _vp,
+     _uint, _uint, _sz, _vp)
+
+# The reference-signature entry points, usable wherever a CompressFunc_t is.
+compress_dxt1_fn = CompressFunc(("goofy_hip_compress_dxt1", _lib))
+compress_etc1_fn = CompressFunc(("goofy_hip_compress_etc1", _lib))
+
+
+def last_error() -> str:
+    return _lib.goofy_hip_last_error().decode()
+
+
+def version() -> str:
+    return _lib.goofy_hip_version().decode()
+
+
+def device_count() -> int:
+    return int(_lib.goofy_hip_device_count())
+
+
+def _check(status: int) -> int:
+    if status != OK:
+        raise GoofyError(status, last_error() or _lib.goofy_hip_status_string(status).decode())
+    return status
+
+
+def _u8(a) -> ctypes.POINTER(ctypes.c_uint8):
+    if a is None:
+        return ctypes.cast(None, _u8p)
+    return a.ctypes.data_as(_u8p)
+
+
+def _ptr(x) -> int | None:
+    """Device address from an int or a torch tensor (data_ptr)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return int(x.data_ptr())
+
+
+def _stream(s) -> int | None:
+    if s is None or isinstance(s, int):
+        return s
+    return int(s.cuda_stream)  # torch.cuda.Stream
+
+
+# ---- drop-in (host memory) ------------------------------------------------------
+
+def compressDXT1(result: np.ndarray, input: np.ndarray, width: int, height: int, stride: int) -> int:
+    """goofy::compressDXT1 (GoofyTC/goofy_tc.h:1497): returns 0 / -1 / -2 like the reference."""
+    return int(_lib.goofy_hip_compress_dxt1(_u8(result), _u8(input), width, height, stride))
+
+
+def compressETC1(result: np.ndarray, input: np.ndarray, width: int, height: int, stride: int) -> int:
+    """goofy::compressETC1 (GoofyTC/goofy_tc.h:1528): returns 0 / -1 / -2 like the reference."""
+    return int(_lib.goofy_hip_compress_etc1(_u8(result), _u8(input), width, height, stride))
+
+
+def compress(fmt: int, image: np.ndarray, width: int, height: int, stride: int | None = None) -> np.ndarray:
+    """Encode a host RGBA8 image; returns the (w/4)*(h/4)*8 block bytes.  Raises on error."""
+    stride = width * 4 if stride is None else stride
+    out = np.empty(max(1, (width // 4) * (height // 4) * 8), dtype=np.uint8)
+    _check(_lib.goofy_hip_compress(fmt, _u8(out), _u8(image), width, height, stride))
+    return out[: (width // 4) * (height // 4) * 8]
+
+
+# ---- device memory ----------------------------------------------------------------
+
+def compress_device(fmt: int, dst, src, width: int, height: int, pitch: int, stream=None,
+                    config: LaunchConfig | dict | None = None) -> None:
+    """Enqueue the encode of a device-resident image (dst/src: device addresses or torch tensors)."""
+    if config is None:
+        _check(_lib.goofy_hip_compress_device(fmt, _ptr(dst), _ptr(src), width, height, pitch, _stream(stream)))
+    else:
+        cfg = config if isinstance(config, LaunchConfig) else LaunchConfig(**config)
+        _check(_lib.goofy_hip_compress_device_cfg(fmt, _ptr(dst), _ptr(src), width, height, pitch,
+                                                  _stream(stream), ctypes.byref(cfg)))
+
+
+class Batch:
+    """Many device-resident textures encoded by one persistent-kernel launch (hipGraph replay)."""
+
+    def __init__(self, textures: Sequence[dict], config: LaunchConfig | dict | None = None):
+        arr = (TextureDesc * max(1, len(textures)))()
+        for i, t in enumerate(textures):
+            arr[i] = TextureDesc(_ptr(t["src"]), _ptr(t["dst"]), t["width"], t["height"],
+                                 t.get("pitch", t["width"] * 4), t["format"])
+        h = _vp()
+        cfg = None
+        if config is not None:
+            cfg = config if isinstance(config, LaunchConfig) else LaunchConfig(**config)
+        _check(_lib.goofy_batch_create(arr, len(textures), ctypes.byref(cfg) if cfg else None, ctypes.byref(h)))
+        self._h = h
+        self._keep = arr
+
+    def launch(self, stream=None) -> None:
+        _check(_lib.goofy_batch_launch(self._h, _stream(stream)))
+
+    def launch_per_texture(self, stream=None) -> None:
+        _check(_lib.goofy_batch_launch_per_texture(self._h, _stream(stream)))
+
+    def autotune(self, stream=None, reps: int = 3) -> tuple[dict, float]:
+        cfg = LaunchConfig()
+        ms = ctypes.c_float()
+        _check(_lib.goofy_batch_autotune(self._h, _stream(stream), reps, ctypes.byref(cfg), ctypes.byref(ms)))
+        return cfg.as_dict(), float(ms.value)
+
+    @property
+    def config(self) -> dict:
+        cfg = LaunchConfig()
+        _check(_lib.goofy_batch_get_config(self._h, ctypes.byref(cfg)))
+        return cfg.as_dict()
+
+    @config.setter
+    def config(self, value: dict) -> None:
+        cfg = LaunchConfig(**value)
+        _check(_lib.goofy_batch_set_config(self._h, ctypes.byref(cfg)))
+
+    def info(self) -> dict:
+        v = [ctypes.c_uint64() for _ in range(4)]
+        _check(_lib.goofy_batch_info(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("textures", "blocks", "tiles", "pixels"), (int(x.value) for x in v)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.goofy_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- synthetic inputs -----------------------------------------------------------------
+
+WAVES, STRESS = 0, 1
+
+
+def synth_host(kind: int, seed: int, texture_id: int, noise: int, width: int, height: int,
+               pitch: int | None = None, threads: int = 0, fill: int = 0) -> np.ndarray:
+    """Seeded synthetic RGBA8 texture in host memory; returns a flat uint8 array of pitch*height
+    bytes (256-byte aligned).  Padding bytes beyond 4*width hold `fill`."""
+    pitch = width * 4 if pitch is None else pitch
+    raw = np.empty(pitch * height + 256, dtype=np.uint8)
+    off = (-raw.ctypes.data) % 256
+    img = raw[off:off + pitch * height]
+    if pitch != width * 4:
+        img.fill(fill)
+    _check(_lib.goofy_synth_fill_host(kind, seed, texture_id, noise, _u8(img), width, height, pitch, threads))
+    return img
+
+
+def synth_device(kind: int, seed: int, texture_id: int, noise: int, dst, width: int, height: int,
+                 pitch: int | None = None, stream=None) -> None:
+    pitch = width * 4 if pitch is None else pitch
+    _check(_lib.goofy_synth_fill_device(kind, seed, texture_id, noise, _ptr(dst), width, height, pitch,
+                                        _stream(stream)))
+
+
+__all__ = [
+    "DXT1", "ETC1", "FORMATS", "GoofyError", "LaunchConfig", "TextureDesc", "CompressFunc", "Batch",
+    "compressDXT1", "compressETC1", "compress", "compress_device", "compress_dxt1_fn", "compress_etc1_fn",
+    "synth_host", "synth_device", "last_error", "version", "device_count", "WAVES", "STRESS",
+]

@@ -1,0 +1,225 @@
+// block_codec.h -- the per-4x4-block DXT1 / ETC1S encoder for gfx950.
+//
+// One lane owns one block: its 16 RGBA8 pixels sit in 16 VGPRs (dword p =
+// pixel 4*row+col, R in bits 0..7) and every step below is lane-local byte
+// arithmetic.  This replaces the 16-byte SSE2 lanes of goofySimdEncode
+// (GoofyTC/goofy_tc.h:1069-1494), which encode 4 blocks per call and spend
+// most of their instructions on transposes (transposeAs4x4 :263-276,
+// deinterleaveRGB :278-299, transposeAs4x4x4 :327-358); here there is no
+// transpose at all: v_perm_b32 gathers bytes, v_lerp_u8 is the per-byte
+// rounding average (_mm_avg_epu8), packed 16-bit min/max give the bounding
+// box, and per-byte comparisons come out of the same rounding average.
+//
+// Integer semantics (all values u8, avg(a,b) = (a+b+1)>>1):
+//   Y(r,g,b) = avg(avg(r,b),g)                       goofy_tc.h:1166,1205
+//   range = max(Y(max)-Y(min), 8), mid = avg(Y(min),Y(max))   :1176-1179
+//   qt = avg(avg(range,0),0) + avg(avg(avg(range,0),0),0)     :1184-1190
+//   gez(p) = Y(p) >= mid ; near(p) = |Y(p)-mid| < qt          :1212-1224
+//     (the header clamps |Y-mid| to 127 before a signed compare; qt <= 96,
+//      so the clamp never changes the outcome)
+//   q5(c) = avg(avg(avg(subsat(c,8),0),0),0) = c ? (c-1)>>3 : 0   :1309
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace goofy_dev {
+
+enum : int { kDXT1 = 0, kETC1 = 1 };
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Per-byte rounding average of two packed words: v_lerp_u8 with a carry-in
+// of one in every byte, i.e. (a+b+1)>>1 per byte.
+__device__ __forceinline__ uint32_t avg4(uint32_t a, uint32_t b) { return __builtin_amdgcn_lerp(a, b, 0x01010101u); }
+
+// v_perm_b32: byte i of the result is byte sel[i] of {hi:lo} (0..3 = lo,
+// 4..7 = hi, 0x0c = zero).
+__device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
+{
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+__device__ __forceinline__ uint32_t pk_min16(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_max16(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+
+// Luma of the colour in byte lanes 0..2 of c (result in byte 0, other bytes
+// are don't-care).
+__device__ __forceinline__ uint32_t luma_b0(uint32_t c)
+{
+    const uint32_t rb = avg4(c, c >> 16);  // byte0 = avg(R,B)
+    return avg4(rb, c >> 8) & 0xFFu;       // byte0 = avg(avg(R,B),G)
+}
+
+// Luma of four pixels, packed: byte i = Y(p_i).
+__device__ __forceinline__ uint32_t luma4(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3)
+{
+    // [R0 R1 G0 G1] and [B0 B1 G0 G1] -> avg -> [avg(R,B)0, avg(R,B)1, G0, G1]
+    const uint32_t q01 = avg4(perm(p1, p0, 0x05010400u), perm(p1, p0, 0x05010602u));
+    const uint32_t q23 = avg4(perm(p3, p2, 0x05010400u), perm(p3, p2, 0x05010602u));
+    // [rb0 rb1 rb2 rb3] and [G0 G1 G2 G3]
+    return avg4(perm(q23, q01, 0x05040100u), perm(q23, q01, 0x07060302u));
+}
+
+// Byte 0 of x copied into all four bytes (v_perm_b32 broadcast; avoids the
+// quarter-rate v_mul_lo_u32 of x * 0x01010101).
+__device__ __forceinline__ uint32_t rep4(uint32_t x) { return perm(0u, x, 0x00000000u); }
+
+// 8-bit channel -> 5-bit, closed form of the header's three-halvings cascade.
+__device__ __forceinline__ uint32_t q5(uint32_t c) { return (c - (c != 0u ? 1u : 0u)) >> 3; }
+
+struct BlockStats {
+    uint32_t mn, mx;      // packed RGBA bounding-box corners
+    uint32_t range, mid, qt;
+};
+
+__device__ __forceinline__ BlockStats block_stats(const uint32_t (&p)[16])
+{
+    // Bounding box: R,B in the two 16-bit halves of p & 0x00ff00ff; G of two
+    // pixels packed into one word with v_perm ([G_a 0 G_b 0]).
+    uint32_t mnrb = p[0] & 0x00FF00FFu, mxrb = mnrb;
+#pragma unroll
+    for (int i = 1; i < 16; ++i) {
+        const uint32_t rb = p[i] & 0x00FF00FFu;
+        mnrb = pk_min16(mnrb, rb);
+        mxrb = pk_max16(mxrb, rb);
+    }
+    uint32_t g = perm(p[1], p[0], 0x0c050c01u);
+    uint32_t mng = g, mxg = g;
+#pragma unroll
+    for (int i = 2; i < 16; i += 2) {
+        g = perm(p[i + 1], p[i], 0x0c050c01u);
+        mng = pk_min16(mng, g);
+        mxg = pk_max16(mxg, g);
+    }
+    mng = pk_min16(mng, mng >> 16) & 0xFFu;
+    mxg = pk_max16(mxg, mxg >> 16) & 0xFFu;
+    BlockStats s;
+    s.mn = mnrb | (mng << 8);
+    s.mx = mxrb | (mxg << 8);
+    const uint32_t ymn = luma_b0(s.mn);
+    const uint32_t ymx = luma_b0(s.mx);  // >= ymn: luma is monotone per channel
+    const uint32_t d = ymx - ymn;
+    s.range = d > 8u ? d : 8u;
+    s.mid = (ymn + ymx + 1u) >> 1;
+    const uint32_t half = (s.range + 1u) >> 1;
+    const uint32_t quarter = (half + 1u) >> 1;
+    const uint32_t eighth = (quarter + 1u) >> 1;
+    s.qt = quarter + eighth;  // <= 96
+    return s;
+}
+
+// Per-byte comparison masks for four packed lumas y4 against the block's mid
+// and threshold: bit 7 of each byte of *gez = (y >= mid), of *near =
+// (|y - mid| < qt).  Uses avg(~a, b) >= 128  <=>  b >= a for bytes a, b.
+__device__ __forceinline__ void compare4(uint32_t y4, uint32_t nmid4, uint32_t nlo4, uint32_t hi4, uint32_t* gez,
+                                         uint32_t* near)
+{
+    *gez = avg4(nmid4, y4);
+    *near = avg4(nlo4, y4) & avg4(~y4, hi4);
+}
+
+// DXT1 (BC1) block: returns {c0 | c1<<16, indices} as two little-endian words.
+__device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16])
+{
+    const BlockStats s = block_stats(p);
+    const uint32_t lo = s.mid + 1u > s.qt ? s.mid + 1u - s.qt : 0u;          // y >= lo
+    const uint32_t hi = s.mid + s.qt - 1u < 255u ? s.mid + s.qt - 1u : 255u;  // y <= hi
+    const uint32_t nmid4 = rep4(~s.mid);
+    const uint32_t nlo4 = rep4(~lo);
+    const uint32_t hi4 = rep4(hi);
+
+    // code byte per pixel: bit0 = y < mid, bit1 = near; row r in word r
+    uint32_t idx = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t y4 = luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]);
+        uint32_t gez, near;
+        compare4(y4, nmid4, nlo4, hi4, &gez, &near);
+        const uint32_t code = ((~gez >> 7) & 0x01010101u) | ((near >> 6) & 0x02020202u);
+        // bytes (c0,c1,c2,c3) with 2-bit codes -> one byte c0|c1<<2|c2<<4|c3<<6
+        uint32_t t = code | (code >> 6);
+        t = t | (t >> 12);
+        idx |= (t & 0xFFu) << (8 * r);
+    }
+    // RGB565 endpoints; green LSB of c0 set so c0 > c1 (4-colour mode)
+    const uint32_t c0 = 0x20u | (q5(s.mx & 0xFFu) << 11) | (q5((s.mx >> 8) & 0xFFu) << 6) | q5((s.mx >> 16) & 0xFFu);
+    const uint32_t c1 = (q5(s.mn & 0xFFu) << 11) | (q5((s.mn >> 8) & 0xFFu) << 6) | q5((s.mn >> 16) & 0xFFu);
+    return make_uint2(c0 | (c1 << 16), idx);
+}
+
+// ETC1 codeword-table index for the brightness range (breakpoints of
+// etc1BrighnessRangeTocontrolByte, GoofyTC/goofy_tc.h:1040-1056).
+__device__ __forceinline__ uint32_t etc1_table(uint32_t range)
+{
+    return (range > 21u) + (range > 43u) + (range > 73u) + (range > 105u) + (range > 151u) + (range > 181u) +
+           (range > 253u);
+}
+
+// ETC1S block: differential mode, zero deltas, one table for both halves,
+// flip = 1; returns the 8 bytes as two little-endian words.
+__device__ __forceinline__ uint2 encode_etc1(const uint32_t (&p)[16])
+{
+    const BlockStats s = block_stats(p);
+    const uint32_t lo = s.mid + 1u > s.qt ? s.mid + 1u - s.qt : 0u;
+    const uint32_t hi = s.mid + s.qt - 1u < 255u ? s.mid + s.qt - 1u : 255u;
+    const uint32_t nmid4 = rep4(~s.mid);
+    const uint32_t nlo4 = rep4(~lo);
+    const uint32_t hi4 = rep4(hi);
+
+    // Selector planes.  Row r's four MSB flags (bit 8c+7) go to bit 8c+r, so
+    // byte c of G/N holds column c's nibble (rows 0..3).
+    uint32_t G = 0, N = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t y4 = luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]);
+        uint32_t gez, near;
+        compare4(y4, nmid4, nlo4, hi4, &gez, &near);
+        G |= (gez & 0x80808080u) >> (7 - r);
+        N |= (near & 0x80808080u) >> (7 - r);
+    }
+    // Word bit j <-> column {2,3,0,1}[j>>2], row j&3 (goofy_tc.h:1375-1395):
+    // fold nibbles (byte0 = col0|col1<<4, byte2 = col2|col3<<4), then put
+    // G's pair in bits 0..15 and N's in 16..31 and invert.
+    G |= G >> 4;
+    N |= N >> 4;
+    const uint32_t w1 = ~perm(N, G, 0x04060002u);
+
+    // Average colour: rows first, then columns (goofy_tc.h:1402-1414).
+    uint32_t col[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) col[x] = avg4(avg4(p[x], p[4 + x]), avg4(p[8 + x], p[12 + x]));
+    const uint32_t avgc = avg4(avg4(col[0], col[1]), avg4(col[2], col[3]));
+    // Shift its brightness onto mid (goofy_tc.h:1431-1449).
+    const uint32_t ya = luma_b0(avgc);
+    const bool up = ya <= s.mid;
+    uint32_t corr = up ? s.mid - ya : ya - s.mid;
+    corr = corr < 127u ? corr : 127u;
+    uint32_t w0 = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t a = (avgc >> (8 * c)) & 0xFFu;
+        uint32_t b = up ? a + corr : (a > corr ? a - corr : 0u);
+        b = b < 255u ? b : 255u;
+        // q5(b) << 3 == subsat(b,1) & 0xf8
+        w0 |= ((b - (b != 0u ? 1u : 0u)) & 0xF8u) << (8 * c);
+    }
+    const uint32_t cw = etc1_table(s.range);
+    w0 |= ((cw << 5) | (cw << 2) | 3u) << 24;
+    return make_uint2(w0, w1);
+}
+
+template <int FMT>
+__device__ __forceinline__ uint2 encode_block(const uint32_t (&p)[16])
+{
+    if constexpr (FMT == kDXT1) return encode_dxt1(p);
+    else return encode_etc1(p);
+}
+
+}  // namespace goofy_dev

@@ -1,0 +1,338 @@
+// capi.cpp -- single-image entry points of libgoofy_hip.so: the drop-in
+// goofy::compressDXT1 / compressETC1 (host memory, synchronous), the device
+// pointer / stream variant, error reporting and the synthetic-input fill.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "goofy_hip.h"
+#include "goofy_tc.h"
+#include "internal.h"
+#include "synth.h"
+
+namespace goofy_impl {
+
+// ---- errors -----------------------------------------------------------------
+
+static thread_local std::string t_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    t_last_error = msg;
+    return code;
+}
+
+int fail_hip(hipError_t e, const char* what)
+{
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? GOOFY_ERR_OUT_OF_MEMORY
+                     : (e == hipErrorNoDevice)                                     ? GOOFY_ERR_NO_DEVICE
+                                                                                   : GOOFY_ERR_HIP;
+    return fail(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void clear_error() { t_last_error.clear(); }
+
+// ---- device -------------------------------------------------------------------
+
+int device_info(DeviceInfo* out)
+{
+    static std::mutex mu;
+    static DeviceInfo cache[64];
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return fail(GOOFY_ERR_NO_DEVICE, "no GPU visible");
+    }
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail_hip(e, "hipGetDevice");
+    if (dev < 0 || dev >= 64) return fail(GOOFY_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (cache[dev].device != dev) {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, dev);
+        if (e != hipSuccess) return fail_hip(e, "hipGetDeviceProperties");
+        cache[dev].num_cus = prop.multiProcessorCount;
+        cache[dev].total_mem = prop.totalGlobalMem;
+        std::snprintf(cache[dev].arch, sizeof(cache[dev].arch), "%s", prop.gcnArchName);
+        cache[dev].device = dev;
+    }
+    *out = cache[dev];
+    return GOOFY_OK;
+}
+
+LaunchCfg default_plain_cfg() { return LaunchCfg{2, 4, 0, 0}; }
+LaunchCfg default_batch_cfg() { return LaunchCfg{2, 4, 16, 0}; }
+
+bool cfg_valid(const LaunchCfg& c)
+{
+    const bool bpl = c.blocks_per_lane == 1 || c.blocks_per_lane == 2 || c.blocks_per_lane == 4;
+    const bool wg = c.wg_waves == 1 || c.wg_waves == 2 || c.wg_waves == 4 || c.wg_waves == 8;
+    return bpl && wg && c.waves_per_cu >= 0 && c.waves_per_cu <= 256 && (c.nontemporal == 0 || c.nontemporal == 1);
+}
+
+// ---- staging contexts for the host-memory drop-in ------------------------------
+//
+// A context is a stream plus grow-only device buffers.  Contexts live in a
+// process-wide pool: a call takes one, uses it, and gives it back, so
+// concurrent callers never share a stream or a buffer, and nothing is freed
+// at thread exit (the HIP runtime may already be gone then).
+
+struct StagingCtx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    void* d_in = nullptr;
+    size_t in_cap = 0;
+    void* d_out = nullptr;
+    size_t out_cap = 0;
+};
+
+class StagingPool {
+public:
+    StagingCtx* acquire(int device)
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (size_t i = 0; i < free_.size(); ++i)
+            if (free_[i]->device == device) {
+                StagingCtx* c = free_[i];
+                free_.erase(free_.begin() + (long)i);
+                return c;
+            }
+        all_.push_back(new StagingCtx());
+        all_.back()->device = device;
+        return all_.back();
+    }
+    void release(StagingCtx* c)
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        free_.push_back(c);
+    }
+
+private:
+    std::mutex mu_;
+    std::vector<StagingCtx*> all_, free_;
+};
+
+static StagingPool& staging_pool()
+{
+    static StagingPool* pool = new StagingPool();  // never destroyed
+    return *pool;
+}
+
+static int ensure(void** buf, size_t* cap, size_t need, const char* what)
+{
+    if (*cap >= need) return GOOFY_OK;
+    if (*buf) {
+        (void)hipFree(*buf);
+        *buf = nullptr;
+        *cap = 0;
+    }
+    const size_t sz = std::max<size_t>(need, 1u << 20);
+    hipError_t e = hipMalloc(buf, sz);
+    if (e != hipSuccess) {
+        *buf = nullptr;
+        return fail_hip(e, what);
+    }
+    *cap = sz;
+    return GOOFY_OK;
+}
+
+static int compress_host(int fmt, unsigned char* result, const unsigned char* input, unsigned width,
+                         unsigned height, unsigned stride)
+{
+    clear_error();
+    if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
+    // Same checks, same order as the reference (GoofyTC/goofy_tc.h:1500-1508).
+    if (width % 16 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 16");
+    if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
+    if (width == 0 || height == 0) return GOOFY_OK;
+    if (!result || !input) return fail(GOOFY_ERR_INVALID, "null buffer");
+    if ((size_t)stride < (size_t)width * 4) return fail(GOOFY_ERR_INVALID, "stride smaller than 4*width");
+    const uint32_t bw = width / 4, bh = height / 4;
+    const uint64_t nblocks = (uint64_t)bw * bh;
+    if (!magic_ok(bw, nblocks) || nblocks > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
+
+    DeviceInfo di;
+    int rc = device_info(&di);
+    if (rc != GOOFY_OK) return rc;
+    StagingCtx* ctx = staging_pool().acquire(di.device);
+    struct Release {
+        StagingCtx* c;
+        ~Release() { staging_pool().release(c); }
+    } rel{ctx};
+    hipError_t e;
+    if (!ctx->stream) {
+        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return fail_hip(e, "hipStreamCreate");
+    }
+    const size_t pitch = (size_t)width * 4;
+    const size_t in_bytes = pitch * height, out_bytes = nblocks * 8;
+    if ((rc = ensure(&ctx->d_in, &ctx->in_cap, in_bytes, "hipMalloc(input staging)")) != GOOFY_OK) return rc;
+    if ((rc = ensure(&ctx->d_out, &ctx->out_cap, out_bytes, "hipMalloc(output staging)")) != GOOFY_OK) return rc;
+
+    e = hipMemcpy2DAsync(ctx->d_in, pitch, input, stride, pitch, height, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy2DAsync(H2D)");
+    e = launch_plain(fmt, default_plain_cfg(), ctx->d_in, pitch, ctx->d_out, bw, (uint32_t)nblocks, ctx->stream);
+    if (e != hipSuccess) return fail_hip(e, "encode kernel launch");
+    e = hipMemcpyAsync(result, ctx->d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpyAsync(D2H)");
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    return GOOFY_OK;
+}
+
+static int compress_device(int fmt, void* d_result, const void* d_input, unsigned width, unsigned height,
+                           size_t pitch, void* stream, const goofy_launch_config* ucfg)
+{
+    clear_error();
+    if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
+    if (width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
+    if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
+    if (width == 0 || height == 0) return GOOFY_OK;
+    if (!d_result || !d_input) return fail(GOOFY_ERR_INVALID, "null buffer");
+    if (pitch < (size_t)width * 4 || pitch % 16 != 0)
+        return fail(GOOFY_ERR_INVALID, "pitch must be >= 4*width and a multiple of 16");
+    if ((uintptr_t)d_input % 16 != 0 || (uintptr_t)d_result % 8 != 0)
+        return fail(GOOFY_ERR_INVALID, "input must be 16-byte and result 8-byte aligned");
+    const uint32_t bw = width / 4, bh = height / 4;
+    const uint64_t nblocks = (uint64_t)bw * bh;
+    if (!magic_ok(bw, nblocks) || nblocks > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
+    LaunchCfg cfg = default_plain_cfg();
+    if (ucfg) {
+        cfg = LaunchCfg{ucfg->blocks_per_lane, ucfg->wg_waves, ucfg->waves_per_cu, ucfg->nontemporal};
+        if (!cfg_valid(cfg)) return fail(GOOFY_ERR_INVALID, "invalid launch config");
+    }
+    hipError_t e = launch_plain(fmt, cfg, d_input, pitch, d_result, bw, (uint32_t)nblocks, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "encode kernel launch");
+    return GOOFY_OK;
+}
+
+}  // namespace goofy_impl
+
+using namespace goofy_impl;
+
+extern "C" {
+
+const char* goofy_hip_last_error(void) { return t_last_error.c_str(); }
+
+const char* goofy_hip_status_string(int s)
+{
+    switch (s) {
+    case GOOFY_OK: return "ok";
+    case GOOFY_ERR_WIDTH: return "bad width";
+    case GOOFY_ERR_HEIGHT: return "bad height";
+    case GOOFY_ERR_HIP: return "HIP runtime error";
+    case GOOFY_ERR_INVALID: return "invalid argument";
+    case GOOFY_ERR_OUT_OF_MEMORY: return "out of memory";
+    case GOOFY_ERR_NO_DEVICE: return "no GPU";
+    case GOOFY_ERR_TOO_LARGE: return "image too large";
+    case GOOFY_ERR_BUSY: return "busy";
+    default: return "unknown status";
+    }
+}
+
+const char* goofy_hip_version(void) { return "goofy_hip 0.1 (gfx950)"; }
+
+int goofy_hip_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int goofy_hip_compress(int format, unsigned char* result, const unsigned char* input, unsigned int width,
+                       unsigned int height, unsigned int stride)
+{
+    return compress_host(format, result, input, width, height, stride);
+}
+
+int goofy_hip_compress_dxt1(unsigned char* result, const unsigned char* input, unsigned int width, unsigned int height,
+                            unsigned int stride)
+{
+    return compress_host(GOOFY_FORMAT_DXT1, result, input, width, height, stride);
+}
+
+int goofy_hip_compress_etc1(unsigned char* result, const unsigned char* input, unsigned int width, unsigned int height,
+                            unsigned int stride)
+{
+    return compress_host(GOOFY_FORMAT_ETC1, result, input, width, height, stride);
+}
+
+int goofy_hip_compress_device(int format, void* d_result, const void* d_input, unsigned int width,
+                              unsigned int height, size_t pitch, void* stream)
+{
+    return compress_device(format, d_result, d_input, width, height, pitch, stream, nullptr);
+}
+
+int goofy_hip_compress_device_cfg(int format, void* d_result, const void* d_input, unsigned int width,
+                                  unsigned int height, size_t pitch, void* stream, const goofy_launch_config* cfg)
+{
+    return compress_device(format, d_result, d_input, width, height, pitch, stream, cfg);
+}
+
+int goofy_synth_fill_host(int kind, uint64_t seed, uint32_t texture_id, uint32_t noise, unsigned char* dst,
+                          unsigned int width, unsigned int height, size_t pitch, int threads)
+{
+    clear_error();
+    if (kind != 0 && kind != 1) return fail(GOOFY_ERR_INVALID, "unknown synth kind");
+    if (width == 0 || height == 0) return GOOFY_OK;
+    if (!dst || pitch < (size_t)width * 4) return fail(GOOFY_ERR_INVALID, "bad destination");
+    const goofy_synth::Params P = goofy_synth::make_params((uint32_t)kind, seed, texture_id, noise);
+    if (threads <= 0) threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    threads = (int)std::min<unsigned>((unsigned)threads, height);
+    auto rows = [&](unsigned y0, unsigned y1) {
+        for (unsigned y = y0; y < y1; ++y) {
+            uint32_t* row = reinterpret_cast<uint32_t*>(dst + (size_t)y * pitch);
+            for (unsigned x = 0; x < width; ++x) {
+                const uint32_t v = goofy_synth::pixel(P, x, y);
+                std::memcpy(row + x, &v, 4);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t)
+        pool.emplace_back(rows, (unsigned)((uint64_t)height * t / threads), (unsigned)((uint64_t)height * (t + 1) / threads));
+    rows(0, (unsigned)((uint64_t)height / threads));
+    for (auto& th : pool) th.join();
+    return GOOFY_OK;
+}
+
+int goofy_synth_fill_device(int kind, uint64_t seed, uint32_t texture_id, uint32_t noise, void* d_dst,
+                            unsigned int width, unsigned int height, size_t pitch, void* stream)
+{
+    clear_error();
+    if (kind != 0 && kind != 1) return fail(GOOFY_ERR_INVALID, "unknown synth kind");
+    if (width == 0 || height == 0) return GOOFY_OK;
+    if (!d_dst || pitch < (size_t)width * 4 || pitch % 4 != 0 || (uintptr_t)d_dst % 4 != 0)
+        return fail(GOOFY_ERR_INVALID, "bad destination");
+    if (height > 65535u) return fail(GOOFY_ERR_TOO_LARGE, "height above 65535");
+    const goofy_synth::Params P = goofy_synth::make_params((uint32_t)kind, seed, texture_id, noise);
+    hipError_t e = launch_synth(&P, d_dst, width, height, pitch, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "synth kernel launch");
+    return GOOFY_OK;
+}
+
+}  // extern "C"
+
+// The drop-in C++ API (include/goofy_tc.h).
+namespace goofy {
+__attribute__((visibility("default"))) int compressDXT1(unsigned char* result, const unsigned char* input, unsigned int width, unsigned int height,
+                 unsigned int stride)
+{
+    return goofy_impl::compress_host(GOOFY_FORMAT_DXT1, result, input, width, height, stride);
+}
+__attribute__((visibility("default"))) int compressETC1(unsigned char* result, const unsigned char* input,
+                                                        unsigned int width, unsigned int height, unsigned int stride)
+{
+    return goofy_impl::compress_host(GOOFY_FORMAT_ETC1, result, input, width, height, stride);
+}
+}  // namespace goofy

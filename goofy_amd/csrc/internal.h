@@ -1,0 +1,74 @@
+// internal.h -- declarations shared by the host-side translation units of
+// libgoofy_hip.so (not installed; the public C-ABI is include/goofy_hip.h).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "goofy_hip.h"
+
+namespace goofy_impl {
+
+// Device-side texture descriptor of the flattened work-list (64 bytes).
+// Texture t owns wave tiles [tile_begin, tile_begin + ceil(nblocks / tile)).
+struct TexDesc {
+    uint64_t src;      // device address of pixel (0,0)
+    uint64_t dst;      // device address of block 0 (8 bytes per block, row-major)
+    uint64_t pitch;    // bytes between pixel rows
+    uint64_t magic;    // ceil(2^40 / bw): block row = (i * magic) >> 40
+    uint32_t bw;       // blocks per row
+    uint32_t nblocks;  // bw * bh
+    uint32_t tile_begin;
+    uint32_t fmt;      // GOOFY_FORMAT_DXT1 / GOOFY_FORMAT_ETC1
+    uint32_t pad[4];
+};
+static_assert(sizeof(TexDesc) == 64, "TexDesc layout");
+
+// Kernel launch configuration (what the autotuner searches).
+struct LaunchCfg {
+    int blocks_per_lane;  // 1, 2 or 4 blocks per lane (tile = 64 * bpl blocks per wave)
+    int wg_waves;         // waves per workgroup: 1, 2, 4 or 8
+    int waves_per_cu;     // persistent grid size (0 = one wave per tile, no persistence)
+    int nontemporal;      // 1 = streaming (non-temporal) loads and stores
+};
+
+inline uint64_t div_magic(uint32_t bw) { return ((1ull << 40) + bw - 1) / bw; }
+
+// True when (i * magic) >> 40 == i / bw for every i < nblocks.
+inline bool magic_ok(uint64_t bw, uint64_t nblocks) { return bw > 0 && nblocks * bw < (1ull << 40) && nblocks / bw < (1ull << 24); }
+
+// Kernel launchers (kernels.hip).
+hipError_t launch_plain(int fmt, const LaunchCfg& cfg, const void* src, size_t pitch, void* dst, uint32_t bw,
+                        uint32_t nblocks, hipStream_t stream);
+hipError_t launch_batch(const LaunchCfg& cfg, const TexDesc* d_descs, uint32_t ntex, uint32_t ntiles, int num_cus,
+                        hipStream_t stream);
+hipError_t launch_synth(const void* params, void* dst, uint32_t width, uint32_t height, size_t pitch,
+                        hipStream_t stream);
+hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width, uint32_t height, size_t pitch,
+                         hipStream_t stream);
+hipError_t launch_sqerr(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width, uint32_t height,
+                        unsigned long long* d_sums, hipStream_t stream);
+
+// Error reporting: every C-ABI function stores a message for its thread.
+int fail(int code, const std::string& msg);
+int fail_hip(hipError_t e, const char* what);
+void clear_error();
+
+// Device properties cached per device.
+struct DeviceInfo {
+    int device = -1;
+    int num_cus = 0;
+    size_t total_mem = 0;
+    char arch[64] = {0};
+};
+int device_info(DeviceInfo* out);  // current device; returns GOOFY_OK or an error code
+
+inline int tile_blocks(const LaunchCfg& c) { return 64 * c.blocks_per_lane; }
+LaunchCfg default_plain_cfg();
+LaunchCfg default_batch_cfg();
+bool cfg_valid(const LaunchCfg& c);
+
+}  // namespace goofy_impl

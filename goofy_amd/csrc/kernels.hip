@@ -1,0 +1,200 @@
+// kernels.hip -- gfx950 kernels of libgoofy_hip.so and their launchers.
+//
+// Work decomposition: one lane encodes one 4x4 block (blocks_per_lane of
+// them with the autotuned shape); one wavefront owns a "tile" of 64*bpl
+// consecutive blocks of one texture in row-major block order.  Lane l of a
+// wave reads rows 0..3 of block (tile + l) as four 16-byte loads, so each
+// load instruction of a wave covers 1 KiB of contiguous pixel row (64 blocks
+// x 16 B) -- fully coalesced without an LDS staging pass -- and the 8-byte
+// results of a wave land in 512 contiguous bytes.
+//
+// This replaces the reference driver loop (GoofyTC/goofy_tc.h:1514-1524,
+// 4 blocks per call) and goofySimdEncode (:1069-1494); the arithmetic of a
+// block is in block_codec.h.
+#include <hip/hip_runtime.h>
+
+#include "block_codec.h"
+#include "internal.h"
+#include "synth.h"
+
+namespace goofy_impl {
+
+using goofy_dev::encode_block;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ u32x4 load16(const uint8_t* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    else return *reinterpret_cast<const u32x4*>(p);
+}
+
+template <bool NT>
+__device__ __forceinline__ void store8(uint8_t* p, uint2 v)
+{
+    const u32x2 w = {v.x, v.y};
+    if constexpr (NT) __builtin_nontemporal_store(w, reinterpret_cast<u32x2*>(p));
+    else *reinterpret_cast<u32x2*>(p) = w;
+}
+
+// Encode the tile of 64*BPL blocks starting at block `first` of a texture.
+// All loads of the tile are issued before any arithmetic so that a wave has
+// 64*BPL*64 bytes in flight.
+template <int FMT, int BPL, bool NT>
+__device__ __forceinline__ void encode_tile(const uint8_t* __restrict__ src, uint64_t pitch, uint8_t* __restrict__ dst,
+                                            uint32_t bw, uint64_t magic, uint32_t nblocks, uint32_t first,
+                                            uint32_t lane)
+{
+    u32x4 px[BPL][4];
+#pragma unroll
+    for (int k = 0; k < BPL; ++k) {
+        const uint32_t i = first + 64u * k + lane;
+        if (i < nblocks) {
+            const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40);
+            const uint32_t bx = i - by * bw;
+            const uint8_t* b = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) px[k][r] = load16<NT>(b + r * pitch);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) px[k][r] = u32x4{0u, 0u, 0u, 0u};
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < BPL; ++k) {
+        const uint32_t i = first + 64u * k + lane;
+        if (i < nblocks) {
+            const uint32_t p[16] = {px[k][0].x, px[k][0].y, px[k][0].z, px[k][0].w, px[k][1].x, px[k][1].y,
+                                    px[k][1].z, px[k][1].w, px[k][2].x, px[k][2].y, px[k][2].z, px[k][2].w,
+                                    px[k][3].x, px[k][3].y, px[k][3].z, px[k][3].w};
+            store8<NT>(dst + (uint64_t)i * 8u, encode_block<FMT>(p));
+        }
+    }
+}
+
+// One image; wave w encodes tile w.
+template <int FMT, int BPL, bool NT>
+__global__ __launch_bounds__(512) void k_encode_plain(const uint8_t* __restrict__ src, uint64_t pitch,
+                                                      uint8_t* __restrict__ dst, uint32_t bw, uint32_t nblocks,
+                                                      uint64_t magic)
+{
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    encode_tile<FMT, BPL, NT>(src, pitch, dst, bw, magic, nblocks, wave * 64u * BPL, threadIdx.x & 63u);
+}
+
+// Many textures: waves stride over the flattened tile list (persistent when
+// the grid is smaller than the list).  The owning texture of a tile is found
+// by a wave-uniform binary search over the descriptors' tile_begin (scalar
+// loads); formats are uniform per tile, so the branch never diverges.
+template <int BPL, bool NT>
+__global__ __launch_bounds__(512) void k_encode_batch(const TexDesc* __restrict__ descs, uint32_t ntex,
+                                                      uint32_t ntiles)
+{
+    const uint32_t wpg = blockDim.x >> 6;
+    const uint32_t nwaves = gridDim.x * wpg;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * wpg + (threadIdx.x >> 6)); t < ntiles;
+         t += nwaves) {
+        uint32_t lo = 0, hi = ntex;
+        while (hi - lo > 1u) {
+            const uint32_t m = (lo + hi) >> 1;
+            if (descs[m].tile_begin <= t) lo = m;
+            else hi = m;
+        }
+        const TexDesc& d = descs[lo];
+        const uint32_t first = (t - d.tile_begin) * 64u * BPL;
+        const uint8_t* src = reinterpret_cast<const uint8_t*>(d.src);
+        uint8_t* dst = reinterpret_cast<uint8_t*>(d.dst);
+        if (d.fmt == 0) encode_tile<0, BPL, NT>(src, d.pitch, dst, d.bw, d.magic, d.nblocks, first, lane);
+        else encode_tile<1, BPL, NT>(src, d.pitch, dst, d.bw, d.magic, d.nblocks, first, lane);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_synth(goofy_synth::Params P, uint8_t* __restrict__ dst, uint32_t width,
+                                               uint32_t height, uint64_t pitch)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t y = blockIdx.y;
+    if (x >= width || y >= height) return;
+    *reinterpret_cast<uint32_t*>(dst + (uint64_t)y * pitch + (uint64_t)x * 4u) = goofy_synth::pixel(P, x, y);
+}
+
+// ---- launchers ---------------------------------------------------------------
+
+template <int FMT, int BPL>
+static hipError_t plain_nt(bool nt, dim3 grid, dim3 block, const void* src, size_t pitch, void* dst, uint32_t bw,
+                           uint32_t nblocks, uint64_t magic, hipStream_t s)
+{
+    if (nt)
+        hipLaunchKernelGGL((k_encode_plain<FMT, BPL, true>), grid, block, 0, s, (const uint8_t*)src, (uint64_t)pitch,
+                           (uint8_t*)dst, bw, nblocks, magic);
+    else
+        hipLaunchKernelGGL((k_encode_plain<FMT, BPL, false>), grid, block, 0, s, (const uint8_t*)src,
+                           (uint64_t)pitch, (uint8_t*)dst, bw, nblocks, magic);
+    return hipGetLastError();
+}
+
+template <int FMT>
+static hipError_t plain_bpl(const LaunchCfg& c, dim3 grid, dim3 block, const void* src, size_t pitch, void* dst,
+                            uint32_t bw, uint32_t nblocks, uint64_t magic, hipStream_t s)
+{
+    switch (c.blocks_per_lane) {
+    case 1: return plain_nt<FMT, 1>(c.nontemporal, grid, block, src, pitch, dst, bw, nblocks, magic, s);
+    case 2: return plain_nt<FMT, 2>(c.nontemporal, grid, block, src, pitch, dst, bw, nblocks, magic, s);
+    default: return plain_nt<FMT, 4>(c.nontemporal, grid, block, src, pitch, dst, bw, nblocks, magic, s);
+    }
+}
+
+hipError_t launch_plain(int fmt, const LaunchCfg& c, const void* src, size_t pitch, void* dst, uint32_t bw,
+                        uint32_t nblocks, hipStream_t stream)
+{
+    if (nblocks == 0) return hipSuccess;
+    const uint32_t tile = 64u * (uint32_t)c.blocks_per_lane;
+    const uint32_t ntiles = (nblocks + tile - 1) / tile;
+    const uint32_t wg = (uint32_t)c.wg_waves;
+    const dim3 grid((ntiles + wg - 1) / wg), block(64u * wg);
+    const uint64_t magic = div_magic(bw);
+    if (fmt == GOOFY_FORMAT_DXT1) return plain_bpl<0>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream);
+    return plain_bpl<1>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream);
+}
+
+template <int BPL>
+static hipError_t batch_nt(bool nt, dim3 grid, dim3 block, const TexDesc* d, uint32_t ntex, uint32_t ntiles,
+                           hipStream_t s)
+{
+    if (nt) hipLaunchKernelGGL((k_encode_batch<BPL, true>), grid, block, 0, s, d, ntex, ntiles);
+    else hipLaunchKernelGGL((k_encode_batch<BPL, false>), grid, block, 0, s, d, ntex, ntiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(const LaunchCfg& c, const TexDesc* d_descs, uint32_t ntex, uint32_t ntiles, int num_cus,
+                        hipStream_t stream)
+{
+    if (ntiles == 0 || ntex == 0) return hipSuccess;
+    const uint32_t wg = (uint32_t)c.wg_waves;
+    uint64_t waves = ntiles;
+    if (c.waves_per_cu > 0) {
+        const uint64_t persistent = (uint64_t)num_cus * (uint64_t)c.waves_per_cu;
+        if (persistent < waves) waves = persistent;
+    }
+    const dim3 grid((uint32_t)((waves + wg - 1) / wg)), block(64u * wg);
+    switch (c.blocks_per_lane) {
+    case 1: return batch_nt<1>(c.nontemporal, grid, block, d_descs, ntex, ntiles, stream);
+    case 2: return batch_nt<2>(c.nontemporal, grid, block, d_descs, ntex, ntiles, stream);
+    default: return batch_nt<4>(c.nontemporal, grid, block, d_descs, ntex, ntiles, stream);
+    }
+}
+
+hipError_t launch_synth(const void* params, void* dst, uint32_t width, uint32_t height, size_t pitch,
+                        hipStream_t stream)
+{
+    if (width == 0 || height == 0) return hipSuccess;
+    const goofy_synth::Params P = *static_cast<const goofy_synth::Params*>(params);
+    const dim3 grid((width + 255u) / 256u, height), block(256);
+    hipLaunchKernelGGL(k_synth, grid, block, 0, stream, P, (uint8_t*)dst, width, height, (uint64_t)pitch);
+    return hipGetLastError();
+}
+
+}  // namespace goofy_impl

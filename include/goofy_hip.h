@@ -1,0 +1,135 @@
+/*
+ * goofy_hip.h -- C-ABI of libgoofy_hip.so, the MI355X (gfx950) Goofy block
+ * encoder.  Plain pointers and sizes only; every function returns GOOFY_OK (0)
+ * or a negative goofy_status code, and goofy_hip_last_error() gives the
+ * message of the calling thread's last failure.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   goofy::compressDXT1 / goofy::compressETC1   GoofyTC/goofy_tc.h:10-13, :1497-1557
+ *   the per-4-block kernel goofySimdEncode<C>   GoofyTC/goofy_tc.h:1069-1494
+ *   the harness plugin type CompressFunc_t      Src/main.cpp:644
+ */
+#ifndef GOOFY_HIP_H
+#define GOOFY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GOOFY_HIP_API __attribute__((visibility("default")))
+
+typedef enum goofy_format { GOOFY_FORMAT_DXT1 = 0, GOOFY_FORMAT_ETC1 = 1 } goofy_format;
+
+typedef enum goofy_status {
+    GOOFY_OK = 0,
+    GOOFY_ERR_WIDTH = -1,          /* width % 16 != 0 (drop-in) or width % 4 != 0 (device/batch API) */
+    GOOFY_ERR_HEIGHT = -2,         /* height % 4 != 0 */
+    GOOFY_ERR_HIP = -3,            /* HIP runtime error (message has the HIP error string) */
+    GOOFY_ERR_INVALID = -4,        /* null pointer, bad format, stride/pitch too small or misaligned, bad config */
+    GOOFY_ERR_OUT_OF_MEMORY = -5,  /* device or pinned allocation failed */
+    GOOFY_ERR_NO_DEVICE = -6,      /* no GPU visible */
+    GOOFY_ERR_TOO_LARGE = -7,      /* image beyond the kernel's index range (blocks * blocks-per-row >= 2^40) */
+    GOOFY_ERR_BUSY = -8            /* serving: admission refused (budget or queue full) */
+} goofy_status;
+
+/* Message for the calling thread's most recent failure ("" if none). */
+GOOFY_HIP_API const char* goofy_hip_last_error(void);
+GOOFY_HIP_API const char* goofy_hip_status_string(int status);
+/* Library version string and the gfx target it was built for. */
+GOOFY_HIP_API const char* goofy_hip_version(void);
+/* Number of visible GPUs (0 when none; never fails). */
+GOOFY_HIP_API int goofy_hip_device_count(void);
+
+/* ---- drop-in, host memory ----------------------------------------------
+ * Same contract as goofy::compressDXT1/compressETC1 (GoofyTC/goofy_tc.h:1497-1557):
+ * host `input` RGBA8 with `stride` bytes per row, host `result` receives
+ * (width/4)*(height/4)*8 tightly packed bytes.  -1 if width % 16, then -2 if
+ * height % 4; width or height 0 writes nothing and returns 0.  Synchronous:
+ * stages through device memory on a per-thread stream and returns when
+ * `result` holds the blocks.  Thread-safe. */
+GOOFY_HIP_API int goofy_hip_compress(int format, unsigned char* result, const unsigned char* input,
+                                     unsigned int width, unsigned int height, unsigned int stride);
+/* The same two entry points with the exact CompressFunc_t signature
+ * (Src/main.cpp:644), so a harness can swap them for the reference's. */
+GOOFY_HIP_API int goofy_hip_compress_dxt1(unsigned char* result, const unsigned char* input, unsigned int width,
+                                          unsigned int height, unsigned int stride);
+GOOFY_HIP_API int goofy_hip_compress_etc1(unsigned char* result, const unsigned char* input, unsigned int width,
+                                          unsigned int height, unsigned int stride);
+
+/* ---- device memory, asynchronous ------------------------------------------
+ * Encodes a device-resident RGBA8 image into device-resident blocks on
+ * `stream` (a hipStream_t, NULL = default stream) and returns after the
+ * launch.  Per-block semantics: width and height multiples of 4 (width % 16
+ * is NOT required), `pitch` >= 4*width and a multiple of 16, `d_input`
+ * 16-byte aligned, `d_result` 8-byte aligned.  Output is bit-identical to the
+ * reference on every block. */
+GOOFY_HIP_API int goofy_hip_compress_device(int format, void* d_result, const void* d_input, unsigned int width,
+                                            unsigned int height, size_t pitch, void* stream);
+
+/* Launch shape used by the device and batch entry points. */
+typedef struct goofy_launch_config {
+    int blocks_per_lane; /* 1, 2 or 4 */
+    int wg_waves;        /* waves per workgroup: 1, 2, 4 or 8 */
+    int waves_per_cu;    /* persistent grid: waves per CU (0 = one wave per tile) */
+    int nontemporal;     /* 1 = streaming loads/stores */
+} goofy_launch_config;
+
+/* As goofy_hip_compress_device with an explicit launch shape (NULL = default). */
+GOOFY_HIP_API int goofy_hip_compress_device_cfg(int format, void* d_result, const void* d_input, unsigned int width,
+                                                unsigned int height, size_t pitch, void* stream,
+                                                const goofy_launch_config* cfg);
+
+/* ---- batched encode: many textures, one launch -----------------------------
+ * A plan flattens the block work-lists of n device-resident textures (each
+ * its own size, pitch and format) into one wave-tile list and encodes all of
+ * them with one persistent kernel launch, captured once into a hipGraph. */
+typedef struct goofy_texture_desc {
+    const void* src;     /* device RGBA8, 16-byte aligned */
+    void* dst;           /* device, (width/4)*(height/4)*8 bytes */
+    unsigned int width;  /* multiple of 4 */
+    unsigned int height; /* multiple of 4 */
+    size_t pitch;        /* bytes between rows: >= 4*width, multiple of 16 */
+    int format;          /* goofy_format */
+} goofy_texture_desc;
+
+typedef struct goofy_batch goofy_batch;
+
+/* cfg NULL = library default; the plan keeps its own device copy of the
+ * descriptors (the textures themselves stay owned by the caller). */
+GOOFY_HIP_API int goofy_batch_create(const goofy_texture_desc* textures, size_t count,
+                                     const goofy_launch_config* cfg, goofy_batch** out);
+/* Enqueue the whole batch on `stream` (replays the captured graph). */
+GOOFY_HIP_API int goofy_batch_launch(goofy_batch* batch, void* stream);
+/* Same work as individual launches (one plain kernel per texture), for comparison. */
+GOOFY_HIP_API int goofy_batch_launch_per_texture(goofy_batch* batch, void* stream);
+/* Time the candidate launch shapes on this batch (HIP events, `reps` runs
+ * each), keep the fastest in the plan and the process-wide tuning cache, and
+ * report it.  `best_ms` may be NULL. */
+GOOFY_HIP_API int goofy_batch_autotune(goofy_batch* batch, void* stream, int reps, goofy_launch_config* chosen,
+                                       float* best_ms);
+GOOFY_HIP_API int goofy_batch_get_config(const goofy_batch* batch, goofy_launch_config* cfg);
+GOOFY_HIP_API int goofy_batch_set_config(goofy_batch* batch, const goofy_launch_config* cfg);
+/* Totals of the plan: textures, 4x4 blocks, wave tiles, pixels. */
+GOOFY_HIP_API int goofy_batch_info(const goofy_batch* batch, uint64_t* textures, uint64_t* blocks, uint64_t* tiles,
+                                   uint64_t* pixels);
+GOOFY_HIP_API int goofy_batch_destroy(goofy_batch* batch);
+
+/* ---- synthetic inputs ------------------------------------------------------
+ * Seeded, integer-only texture generator (host and device give identical
+ * bytes).  kind 0: three 2-D sinusoids per channel + uniform noise in
+ * [-noise, noise]; kind 1: 64x64 stress regions (constant, checker,
+ * saturated primaries, noise, normal map).  Alpha is 255; pitch padding is
+ * left untouched. */
+GOOFY_HIP_API int goofy_synth_fill_host(int kind, uint64_t seed, uint32_t texture_id, uint32_t noise,
+                                        unsigned char* dst, unsigned int width, unsigned int height, size_t pitch,
+                                        int threads);
+GOOFY_HIP_API int goofy_synth_fill_device(int kind, uint64_t seed, uint32_t texture_id, uint32_t noise, void* d_dst,
+                                          unsigned int width, unsigned int height, size_t pitch, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

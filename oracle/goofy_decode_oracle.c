@@ -1,0 +1,159 @@
+/*
+ * goofy_decode_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Independent BC1 and ETC1 block decoders, written from the public format
+ * definitions (BC1: two RGB565 endpoints, 2-bit indices LSB-first per pixel
+ * in row-major order; ETC1: 64-bit big-endian word, individual/differential
+ * base colours, eight intensity tables, MSB/LSB selector planes in
+ * column-major pixel order).  The rounding of the BC1 interpolants follows
+ * the decoder the reference's quality harness uses
+ * (Src/decoder.cpp:819-871: expand 5/6-bit by bit replication, thirds by
+ * truncating division), and the ETC1 tables follow Src/decoder.cpp:388-410.
+ *
+ * Used for round-trip invariants on the encoder (every decoded colour lies in
+ * the palette the encoder picked) and as the oracle of the GPU decoders.
+ * Also holds the multi-threaded striped driver of the encoder oracle.
+ */
+#include <pthread.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "goofy_oracle.h"
+
+static inline int clamp255(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+
+void goofy_oracle_decode_bc1(const uint8_t* block, uint8_t* out)
+{
+    const unsigned c0 = (unsigned)block[0] | ((unsigned)block[1] << 8);
+    const unsigned c1 = (unsigned)block[2] | ((unsigned)block[3] << 8);
+    int pal[4][4];
+    const unsigned ep[2] = {c0, c1};
+    for (int e = 0; e < 2; ++e) {
+        const unsigned r = (ep[e] >> 11) & 31, g = (ep[e] >> 5) & 63, b = ep[e] & 31;
+        pal[e][0] = (int)((r << 3) | (r >> 2));
+        pal[e][1] = (int)((g << 2) | (g >> 4));
+        pal[e][2] = (int)((b << 3) | (b >> 2));
+        pal[e][3] = 255;
+    }
+    for (int c = 0; c < 3; ++c) {
+        if (c0 > c1) {
+            pal[2][c] = (2 * pal[0][c] + pal[1][c]) / 3;
+            pal[3][c] = (pal[0][c] + 2 * pal[1][c]) / 3;
+        } else {
+            pal[2][c] = (pal[0][c] + pal[1][c]) / 2;
+            pal[3][c] = 0;
+        }
+    }
+    pal[2][3] = 255;
+    pal[3][3] = (c0 > c1) ? 255 : 0;
+    const uint32_t idx = (uint32_t)block[4] | ((uint32_t)block[5] << 8) | ((uint32_t)block[6] << 16) |
+                         ((uint32_t)block[7] << 24);
+    for (int p = 0; p < 16; ++p) {
+        const int k = (int)((idx >> (2 * p)) & 3u);
+        for (int c = 0; c < 4; ++c) out[4 * p + c] = (uint8_t)pal[k][c];
+    }
+}
+
+static const int kEtc1Tables[8][2] = {{2, 8}, {5, 17}, {9, 29}, {13, 42}, {18, 60}, {24, 80}, {33, 106}, {47, 183}};
+
+void goofy_oracle_decode_etc1(const uint8_t* block, uint8_t* out)
+{
+    const uint32_t hi = ((uint32_t)block[0] << 24) | ((uint32_t)block[1] << 16) | ((uint32_t)block[2] << 8) | block[3];
+    const uint32_t lo = ((uint32_t)block[4] << 24) | ((uint32_t)block[5] << 16) | ((uint32_t)block[6] << 8) | block[7];
+    const int diff = (int)((hi >> 1) & 1u);
+    const int flip = (int)(hi & 1u);
+    int base[2][3];
+    for (int c = 0; c < 3; ++c) {
+        const int shift = 24 - 8 * c; /* R at bits 31..24, G 23..16, B 15..8 */
+        const unsigned field = (hi >> shift) & 0xFFu;
+        if (diff) {
+            const int b5 = (int)(field >> 3);
+            int d = (int)(field & 7u);
+            if (d >= 4) d -= 8;
+            const int b5b = b5 + d; /* out-of-range sums are invalid ETC1 */
+            base[0][c] = (b5 << 3) | (b5 >> 2);
+            base[1][c] = ((b5b & 31) << 3) | ((b5b & 31) >> 2);
+        } else {
+            const int a4 = (int)(field >> 4), b4 = (int)(field & 15u);
+            base[0][c] = (a4 << 4) | a4;
+            base[1][c] = (b4 << 4) | b4;
+        }
+    }
+    const int cw[2] = {(int)((hi >> 5) & 7u), (int)((hi >> 2) & 7u)};
+    for (int y = 0; y < 4; ++y)
+        for (int x = 0; x < 4; ++x) {
+            const int k = 4 * x + y;
+            const int sub = flip ? (y >= 2) : (x >= 2);
+            const int msb = (int)((lo >> (16 + k)) & 1u);
+            const int lsb = (int)((lo >> k) & 1u);
+            const int mag = kEtc1Tables[cw[sub]][lsb];
+            const int delta = msb ? -mag : mag;
+            uint8_t* o = out + 4 * (4 * y + x);
+            for (int c = 0; c < 3; ++c) o[c] = (uint8_t)clamp255(base[sub][c] + delta);
+            o[3] = 255;
+        }
+}
+
+int goofy_oracle_decode_image(int fmt, uint8_t* dst, const uint8_t* blocks, unsigned width, unsigned height)
+{
+    if (width % 4 != 0) return -1;
+    if (height % 4 != 0) return -2;
+    const unsigned bw = width / 4, bh = height / 4;
+    uint8_t px[64];
+    for (unsigned by = 0; by < bh; ++by)
+        for (unsigned bx = 0; bx < bw; ++bx) {
+            const uint8_t* b = blocks + ((size_t)by * bw + bx) * 8;
+            if (fmt == GOOFY_ORACLE_DXT1) goofy_oracle_decode_bc1(b, px);
+            else goofy_oracle_decode_etc1(b, px);
+            for (int r = 0; r < 4; ++r)
+                memcpy(dst + ((size_t)by * 4 + r) * width * 4 + (size_t)bx * 16, px + 16 * r, 16);
+        }
+    return 0;
+}
+
+/* ---- striped multi-threaded encoder driver ------------------------------ */
+
+typedef struct {
+    int fmt;
+    uint8_t* dst;
+    const uint8_t* src;
+    unsigned width, height, row0, row1;
+    size_t stride;
+} stripe_job;
+
+static void* stripe_main(void* arg)
+{
+    const stripe_job* j = (const stripe_job*)arg;
+    goofy_oracle_compress_blocks(j->fmt, j->dst, j->src, j->width, j->height, j->stride, j->row0, j->row1);
+    return NULL;
+}
+
+int goofy_oracle_compress_mt(int fmt, uint8_t* dst, const uint8_t* src, unsigned width, unsigned height,
+                             size_t stride, int nthreads)
+{
+    if (width % 4 != 0) return -1;
+    if (height % 4 != 0) return -2;
+    const unsigned bh = height / 4;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    if ((unsigned)nthreads > bh) nthreads = bh ? (int)bh : 1;
+    pthread_t th[256];
+    stripe_job jobs[256];
+    for (int t = 0; t < nthreads; ++t) {
+        jobs[t].fmt = fmt; jobs[t].dst = dst; jobs[t].src = src;
+        jobs[t].width = width; jobs[t].height = height; jobs[t].stride = stride;
+        jobs[t].row0 = (unsigned)((uint64_t)bh * t / nthreads);
+        jobs[t].row1 = (unsigned)((uint64_t)bh * (t + 1) / nthreads);
+    }
+    int started = 0;
+    for (int t = 1; t < nthreads; ++t) {
+        if (pthread_create(&th[t], NULL, stripe_main, &jobs[t]) != 0) break;
+        started = t;
+    }
+    stripe_main(&jobs[0]);
+    for (int t = 1; t <= started; ++t) pthread_join(th[t], NULL);
+    /* any stripe whose thread could not start runs here */
+    for (int t = started + 1; t < nthreads; ++t) stripe_main(&jobs[t]);
+    return 0;
+}

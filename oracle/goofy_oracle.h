@@ -1,0 +1,53 @@
+/*
+ * goofy_oracle.h -- TEST INFRASTRUCTURE ONLY: the CPU oracle of the Goofy
+ * DXT1 / ETC1S block encoder (restatement of GoofyTC/goofy_tc.h) plus an
+ * independent BC1 / ETC1 block decoder used for round-trip invariants.
+ * Only tests, the smoke check and the benchmark's CPU-baseline leg use it.
+ */
+#ifndef GOOFY_ORACLE_H
+#define GOOFY_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { GOOFY_ORACLE_DXT1 = 0, GOOFY_ORACLE_ETC1 = 1 };
+
+typedef struct goofy_oracle_block_info {
+    uint8_t mn[4], mx[4];      /* per-channel bounding box (RGBA) */
+    uint8_t ymn, ymx;          /* luma of the two box corners */
+    uint8_t range, mid, qt;    /* max(ymx-ymn,8), avg(ymn,ymx), ~0.375*range */
+    uint16_t gez;              /* bit p: Y(p) >= mid   (p = 4*row + col) */
+    uint16_t lqt;              /* bit p: |Y(p)-mid| < qt */
+} goofy_oracle_block_info;
+
+/* One block: px = 16 RGBA8 pixels, row-major (64 bytes); out = 8 bytes. */
+void goofy_oracle_analyse_block(const uint8_t* px, goofy_oracle_block_info* info);
+void goofy_oracle_encode_block(int fmt, const uint8_t* px, uint8_t* out);
+uint8_t goofy_oracle_etc1_control(uint8_t range);
+
+/* Drop-in semantics (width % 16, height % 4; returns 0 / -1 / -2). */
+int goofy_oracle_compress(int fmt, uint8_t* dst, const uint8_t* src, unsigned width, unsigned height,
+                          unsigned stride);
+/* Per-block semantics for any width % 4 == 0, block rows [row0, row1). */
+int goofy_oracle_compress_blocks(int fmt, uint8_t* dst, const uint8_t* src, unsigned width,
+                                 unsigned height, size_t stride, unsigned row0, unsigned row1);
+/* Multi-threaded per-block driver: one stripe of block rows per thread. */
+int goofy_oracle_compress_mt(int fmt, uint8_t* dst, const uint8_t* src, unsigned width,
+                             unsigned height, size_t stride, int nthreads);
+
+/* Independent decoders (BC1 4-colour/3-colour, ETC1 individual and
+ * differential modes).  out = 16 RGBA8 pixels, row-major. */
+void goofy_oracle_decode_bc1(const uint8_t* block, uint8_t* out);
+void goofy_oracle_decode_etc1(const uint8_t* block, uint8_t* out);
+/* Whole images, tightly packed block rows; dst pitch = width*4. */
+int goofy_oracle_decode_image(int fmt, uint8_t* dst, const uint8_t* blocks, unsigned width,
+                              unsigned height);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

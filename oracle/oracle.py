@@ -1,0 +1,160 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes access to the CPU oracle.
+
+* ``Oracle``  -- the C restatement of GoofyTC/goofy_tc.h (oracle/goofy_oracle.c)
+  plus the independent BC1/ETC1 decoders.
+* ``Reference`` -- the reference header compiled from its own source into
+  oracle/_ref/libgoofy_ref.so (present when the reference tree was available
+  at build time; the built file travels with the repository snapshot).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import
+this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "libgoofy_oracle.so"
+REF_SO = HERE / "_ref" / "libgoofy_ref.so"
+
+DXT1, ETC1 = 0, 1
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u8p)
+
+
+class _BlockInfo(ctypes.Structure):
+    _fields_ = [("mn", ctypes.c_uint8 * 4), ("mx", ctypes.c_uint8 * 4),
+                ("ymn", ctypes.c_uint8), ("ymx", ctypes.c_uint8),
+                ("range", ctypes.c_uint8), ("mid", ctypes.c_uint8), ("qt", ctypes.c_uint8),
+                ("gez", ctypes.c_uint16), ("lqt", ctypes.c_uint16)]
+
+
+def aligned_empty(nbytes: int, align: int = 256) -> np.ndarray:
+    """uint8 buffer whose first byte is `align`-aligned."""
+    raw = np.empty(nbytes + align, dtype=np.uint8)
+    off = (-raw.ctypes.data) % align
+    return raw[off:off + nbytes]
+
+
+class Oracle:
+    def __init__(self, path: Path = ORACLE_SO):
+        if not path.exists():
+            raise FileNotFoundError(f"{path} not built (run `make -C oracle`)")
+        lib = ctypes.CDLL(str(path))
+        lib.goofy_oracle_encode_block.argtypes = [ctypes.c_int, _u8p, _u8p]
+        lib.goofy_oracle_analyse_block.argtypes = [_u8p, ctypes.POINTER(_BlockInfo)]
+        lib.goofy_oracle_etc1_control.argtypes = [ctypes.c_uint8]
+        lib.goofy_oracle_etc1_control.restype = ctypes.c_uint8
+        lib.goofy_oracle_compress.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint]
+        lib.goofy_oracle_compress_blocks.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.c_uint, ctypes.c_uint,
+                                                     ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint]
+        lib.goofy_oracle_compress_mt.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.c_uint, ctypes.c_uint,
+                                                 ctypes.c_size_t, ctypes.c_int]
+        lib.goofy_oracle_decode_bc1.argtypes = [_u8p, _u8p]
+        lib.goofy_oracle_decode_etc1.argtypes = [_u8p, _u8p]
+        lib.goofy_oracle_decode_image.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.c_uint, ctypes.c_uint]
+        self.lib = lib
+
+    # -- blocks ---------------------------------------------------------------
+    def encode_blocks(self, fmt: int, blocks: np.ndarray) -> np.ndarray:
+        """blocks: (n, 64) uint8 (4 rows x 4 RGBA pixels) -> (n, 8) uint8."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)
+        n = blocks.shape[0]
+        # lay blocks side by side in a 4-row image and use the block driver
+        img = np.ascontiguousarray(blocks.reshape(n, 4, 16).transpose(1, 0, 2)).reshape(4, n * 16)
+        out = np.zeros((n, 8), dtype=np.uint8)
+        if n:
+            rc = self.lib.goofy_oracle_compress_blocks(fmt, _ptr(out), _ptr(img), n * 4, 4, n * 16, 0, 1)
+            assert rc == 0
+        return out
+
+    def analyse(self, block: np.ndarray) -> dict:
+        b = np.ascontiguousarray(block, dtype=np.uint8).reshape(64)
+        info = _BlockInfo()
+        self.lib.goofy_oracle_analyse_block(_ptr(b), ctypes.byref(info))
+        return {"mn": bytes(info.mn), "mx": bytes(info.mx), "ymn": info.ymn, "ymx": info.ymx,
+                "range": info.range, "mid": info.mid, "qt": info.qt, "gez": info.gez, "lqt": info.lqt}
+
+    def etc1_control(self, rng: int) -> int:
+        return int(self.lib.goofy_oracle_etc1_control(rng))
+
+    # -- images -----------------------------------------------------------------
+    def compress(self, fmt: int, img: np.ndarray, width: int, height: int, stride: int) -> tuple[int, np.ndarray]:
+        """Drop-in semantics (-1 for width % 16, -2 for height % 4)."""
+        out = np.zeros(max(1, (width // 4) * (height // 4) * 8), dtype=np.uint8)
+        rc = self.lib.goofy_oracle_compress(fmt, _ptr(out), _ptr(img), width, height, stride)
+        return rc, out[: (width // 4) * (height // 4) * 8]
+
+    def compress_blocks(self, fmt: int, img: np.ndarray, width: int, height: int, stride: int,
+                        threads: int = 0) -> np.ndarray:
+        """Per-block semantics for width % 4 == 0 (multi-threaded)."""
+        n = (width // 4) * (height // 4) * 8
+        out = np.zeros(max(1, n), dtype=np.uint8)
+        threads = threads or min(16, os.cpu_count() or 1)
+        rc = self.lib.goofy_oracle_compress_mt(fmt, _ptr(out), _ptr(img), width, height, stride, threads)
+        if rc != 0:
+            raise ValueError(f"oracle rejected {width}x{height}: {rc}")
+        return out[:n]
+
+    def decode_blocks(self, fmt: int, blocks: np.ndarray) -> np.ndarray:
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 8)
+        out = np.zeros((blocks.shape[0], 64), dtype=np.uint8)
+        fn = self.lib.goofy_oracle_decode_bc1 if fmt == DXT1 else self.lib.goofy_oracle_decode_etc1
+        for i in range(blocks.shape[0]):
+            fn(_ptr(blocks[i]), _ptr(out[i]))
+        return out
+
+    def decode_image(self, fmt: int, blocks: np.ndarray, width: int, height: int) -> np.ndarray:
+        out = np.zeros(width * height * 4, dtype=np.uint8)
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        rc = self.lib.goofy_oracle_decode_image(fmt, _ptr(out), _ptr(blocks), width, height)
+        assert rc == 0
+        return out
+
+
+class Reference:
+    """The reference encoder (goofy_tc.h compiled from its own source)."""
+
+    def __init__(self, path: Path = REF_SO):
+        if not path.exists():
+            raise FileNotFoundError(f"{path} not built (reference tree absent at build time)")
+        lib = ctypes.CDLL(str(path))
+        lib.goofy_ref_compress.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint]
+        lib.goofy_ref_compress_mt.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.c_uint, ctypes.c_uint,
+                                              ctypes.c_uint, ctypes.c_int]
+        lib.goofy_ref_encode_blocks.argtypes = [ctypes.c_int, _u8p, ctypes.c_size_t, _u8p]
+        self.lib = lib
+
+    def encode_blocks(self, fmt: int, blocks: np.ndarray) -> np.ndarray:
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)
+        out = np.zeros((blocks.shape[0], 8), dtype=np.uint8)
+        if blocks.shape[0]:
+            rc = self.lib.goofy_ref_encode_blocks(fmt, _ptr(blocks), blocks.shape[0], _ptr(out))
+            assert rc == 0
+        return out
+
+    def compress(self, fmt: int, img: np.ndarray, width: int, height: int, stride: int) -> tuple[int, np.ndarray]:
+        out = np.zeros(max(1, (width // 4) * (height // 4) * 8), dtype=np.uint8)
+        rc = self.lib.goofy_ref_compress(fmt, _ptr(out), _ptr(img), width, height, stride)
+        return rc, out[: (width // 4) * (height // 4) * 8]
+
+    def compress_mt(self, fmt: int, img: np.ndarray, width: int, height: int, stride: int, threads: int,
+                    out: np.ndarray | None = None) -> np.ndarray:
+        n = (width // 4) * (height // 4) * 8
+        if out is None:
+            out = np.zeros(n, dtype=np.uint8)
+        rc = self.lib.goofy_ref_compress_mt(fmt, _ptr(out), _ptr(img), width, height, stride, threads)
+        if rc != 0:
+            raise ValueError(f"reference rejected input: {rc}")
+        return out
+
+
+def have_reference() -> bool:
+    return REF_SO.exists()

@@ -1,0 +1,57 @@
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLDEN = ROOT / "tests" / "golden"
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (HIP kernels run)")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle.oracle import Oracle
+    return Oracle()
+
+
+@pytest.fixture(scope="session")
+def reference():
+    from oracle.oracle import Reference, have_reference
+    if not have_reference():
+        pytest.skip("oracle/_ref (reference compiled from its own source) not built")
+    return Reference()
+
+
+@pytest.fixture(scope="session")
+def golden_blocks():
+    z = np.load(GOLDEN / "blocks.npz")
+    return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
+def golden_img512():
+    z = np.load(GOLDEN / "img512.npz")
+    return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
+def digests():
+    return json.loads((GOLDEN / "digests.json").read_text())
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """The HIP library on a visible GPU, with torch for device memory."""
+    import torch
+    import goofy_amd
+    assert torch.cuda.is_available(), "GPU test run without a visible GPU"
+    assert goofy_amd.device_count() > 0
+    torch.cuda.init()
+    return goofy_amd

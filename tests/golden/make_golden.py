@@ -1,0 +1,164 @@
+"""Write the golden vectors under tests/golden/ from the reference encoder.
+
+The reference (GoofyTC/goofy_tc.h) is compiled from its own source into
+oracle/_ref/libgoofy_ref.so by oracle/Makefile; this script feeds it inputs
+and records what it outputs:
+
+  blocks.npz   -- special and random 4x4 blocks (64 B each) with the
+                  reference's DXT1 and ETC1S outputs (8 B each);
+  img512.npz   -- the full DXT1 / ETC1S outputs of the 512x512 waves image;
+  digests.json -- SHA-256 of inputs and outputs of the 512^2, 8192^2 and the
+                  pitch-padded 16384^2 stress images, and of a 16-texture
+                  batch with widths that are multiples of 4 but not of 16
+                  (encoded by the reference on rows padded to a multiple of
+                  16 pixels, then cropped block-wise).
+
+Inputs come from the library's seeded integer generator; the input digests
+pin that generator too.  Run: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+import goofy_amd  # noqa: E402
+from goofy_amd import workloads as W  # noqa: E402
+from oracle.oracle import Reference, aligned_empty  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def special_blocks(rng: np.random.Generator) -> np.ndarray:
+    blocks = []
+
+    def solid(r, g, b, a=255):
+        return np.tile(np.array([r, g, b, a], np.uint8), 16)
+
+    for c in [(0, 0, 0), (255, 255, 255), (255, 0, 0), (0, 255, 0), (0, 0, 255), (1, 1, 1), (7, 8, 9),
+              (8, 8, 8), (9, 9, 9), (127, 128, 129), (254, 254, 254), (16, 17, 15)]:
+        blocks.append(solid(*c))
+    for _ in range(64):  # random constant colours, random alpha
+        blocks.append(solid(*rng.integers(0, 256, 3), int(rng.integers(0, 256))))
+    # grey two-level blocks covering every luma range 0..255 and several offsets
+    for r in range(256):
+        for lo in {0, (255 - r) // 2, 255 - r}:
+            px = np.full((16, 4), 255, np.uint8)
+            pat = rng.integers(0, 2, 16)
+            pat[0], pat[1] = 0, 1
+            v = np.where(pat == 1, lo + r, lo).astype(np.uint8)
+            px[:, 0] = px[:, 1] = px[:, 2] = v
+            blocks.append(px.reshape(64))
+    # two-colour checkers
+    for _ in range(128):
+        a, b = rng.integers(0, 256, 3), rng.integers(0, 256, 3)
+        px = np.zeros((4, 4, 4), np.uint8)
+        px[..., 3] = 255
+        for y in range(4):
+            for x in range(4):
+                px[y, x, :3] = a if (x + y) % 2 == 0 else b
+        blocks.append(px.reshape(64))
+    # saturated primaries
+    for _ in range(128):
+        px = (rng.integers(0, 2, (16, 4)) * 255).astype(np.uint8)
+        px[:, 3] = 255
+        blocks.append(px.reshape(64))
+    # one outlier pixel in a flat block, at every position
+    for p in range(16):
+        px = np.tile(np.array([100, 120, 140, 255], np.uint8), (16, 1))
+        px[p, :3] = rng.integers(0, 256, 3)
+        blocks.append(px.reshape(64))
+    # alpha noise over identical RGB (alpha must not change the output)
+    base = rng.integers(0, 256, (16, 4)).astype(np.uint8)
+    for _ in range(16):
+        px = base.copy()
+        px[:, 3] = rng.integers(0, 256, 16)
+        blocks.append(px.reshape(64))
+    # mid == qt and near-threshold cases: lumas spread just around mid
+    for mid in range(0, 256, 3):
+        px = np.full((16, 4), 255, np.uint8)
+        v = np.clip(mid + rng.integers(-6, 7, 16), 0, 255).astype(np.uint8)
+        px[:, 0] = px[:, 1] = px[:, 2] = v
+        blocks.append(px.reshape(64))
+    return np.stack(blocks)
+
+
+def random_blocks(rng: np.random.Generator, n: int) -> np.ndarray:
+    out = []
+    out.append(rng.integers(0, 256, (n // 4, 64)).astype(np.uint8))
+    base = rng.integers(0, 256, (n // 4, 1))
+    spread = rng.integers(0, 48, (n // 4, 1))
+    out.append(np.clip(base + rng.integers(-48, 49, (n // 4, 64)) % (spread + 1), 0, 255).astype(np.uint8))
+    # per-channel ramps
+    t = np.linspace(0, 1, 16)[None, :, None]
+    a = rng.integers(0, 256, (n // 4, 1, 4))
+    b = rng.integers(0, 256, (n // 4, 1, 4))
+    out.append(np.clip(a + (b - a) * t + rng.integers(-3, 4, (n // 4, 16, 4)), 0, 255).astype(np.uint8).reshape(-1, 64))
+    # small per-pixel noise around a colour (gradients of natural images)
+    c = rng.integers(20, 236, (n // 4, 1, 4))
+    out.append(np.clip(c + rng.integers(-20, 21, (n // 4, 16, 4)), 0, 255).astype(np.uint8).reshape(-1, 64))
+    return np.concatenate(out)
+
+
+def encode_image_ref(ref: Reference, fmt: int, img: np.ndarray, width: int, height: int, pitch: int) -> np.ndarray:
+    if width % 16 == 0:
+        rc, out = ref.compress(fmt, img, width, height, pitch)
+        assert rc == 0
+        return out
+    # widths that are multiples of 4 only: pad each row to a multiple of 16
+    # pixels (padding blocks are independent of the real ones), encode, crop.
+    wp = (width + 15) // 16 * 16
+    padded = aligned_empty(wp * 4 * height)
+    padded[:] = 0
+    pv = padded.reshape(height, wp * 4)
+    pv[:, : width * 4] = img.reshape(height, pitch)[:, : width * 4]
+    rc, out = ref.compress(fmt, padded, wp, height, wp * 4)
+    assert rc == 0
+    return out.reshape(height // 4, wp // 4, 8)[:, : width // 4].reshape(-1).copy()
+
+
+def main() -> int:
+    ref = Reference()
+    rng = np.random.default_rng(20240601)
+    blocks = np.concatenate([special_blocks(rng), random_blocks(rng, 2048)])
+    dxt1 = ref.encode_blocks(W.DXT1, blocks)
+    etc1 = ref.encode_blocks(W.ETC1, blocks)
+    np.savez_compressed(OUT / "blocks.npz", blocks=blocks, dxt1=dxt1, etc1=etc1)
+    print(f"blocks.npz: {blocks.shape[0]} blocks")
+
+    digests: dict = {"images": {}, "batch": []}
+    for im in (W.IMAGE_512, W.IMAGE_8192, W.STRESS_16384):
+        img = goofy_amd.synth_host(im.kind, im.seed, im.texture_id, im.noise, im.width, im.height, im.pitch)
+        entry = {"width": im.width, "height": im.height, "pitch": im.pitch, "input_sha256": sha(img)}
+        for fmt, name in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+            out = encode_image_ref(ref, fmt, img, im.width, im.height, im.pitch)
+            entry[f"{name}_sha256"] = sha(out)
+            if im is W.IMAGE_512:
+                entry[f"{name}_out"] = out
+        if im is W.IMAGE_512:
+            np.savez_compressed(OUT / "img512.npz", dxt1=entry.pop("dxt1_out"), etc1=entry.pop("etc1_out"))
+        digests["images"][im.name] = entry
+        print(im.name, entry["input_sha256"][:16])
+        del img
+
+    for t in W.batch_textures(n=16, lo=64, hi=512):
+        img = goofy_amd.synth_host(t["kind"], t["seed"], t["texture_id"], t["noise"], t["width"], t["height"])
+        out = encode_image_ref(ref, t["format"], img, t["width"], t["height"], t["width"] * 4)
+        digests["batch"].append({**t, "input_sha256": sha(img), "output_sha256": sha(out)})
+    (OUT / "digests.json").write_text(json.dumps(digests, indent=1) + "\n")
+    print("digests.json written")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

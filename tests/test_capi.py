@@ -1,0 +1,116 @@
+"""The C-ABI library without a GPU: it loads, exports every declared symbol,
+and rejects bad arguments with the reference's codes before touching a device."""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import goofy_amd
+
+ROOT = Path(__file__).resolve().parents[1]
+LIB = ROOT / "goofy_amd" / "libgoofy_hip.so"
+
+
+def declared_functions():
+    names = []
+    for h in sorted((ROOT / "include").glob("*.h")):
+        text = h.read_text()
+        names += re.findall(r"GOOFY_HIP_API\s+[\w\s\*]+?\b(\w+)\s*\(", text)
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 15
+    lib = ctypes.CDLL(str(LIB))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_dropin_cpp_api_exported():
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", str(LIB)], capture_output=True, text=True,
+                         check=True).stdout
+    for sig in ("goofy::compressDXT1(unsigned char*, unsigned char const*, unsigned int, unsigned int, unsigned int)",
+                "goofy::compressETC1(unsigned char*, unsigned char const*, unsigned int, unsigned int, unsigned int)"):
+        assert sig in out
+
+
+def test_dropin_header_matches_reference_signature():
+    text = (ROOT / "include" / "goofy_tc.h").read_text()
+    for name in ("compressDXT1", "compressETC1"):
+        assert re.search(rf"int {name}\(unsigned char\* result, const unsigned char\* input, unsigned int width,"
+                         rf"\s+unsigned int height,\s+unsigned int stride\);", text), name
+
+
+@pytest.mark.parametrize("fn", [goofy_amd.compressDXT1, goofy_amd.compressETC1])
+def test_dropin_return_codes_without_device(fn):
+    out = np.zeros(64, np.uint8)
+    img = np.zeros(64 * 64 * 4, np.uint8)
+    assert fn(out, img, 60, 64, 240) == -1   # width % 16 (GoofyTC/goofy_tc.h:1500)
+    assert fn(out, img, 64, 62, 256) == -2   # height % 4 (:1505)
+    assert fn(out, img, 60, 62, 240) == -1   # width first
+    assert fn(out, img, 0, 64, 0) == 0       # nothing to do
+    assert fn(out, img, 64, 0, 256) == 0
+
+
+def test_compress_func_type_matches_reference_plugin():
+    # CompressFunc_t (Src/main.cpp:644): int(uchar* dst, const uchar* src, uint w, uint h, uint stride)
+    out = np.zeros(64, np.uint8)
+    img = np.zeros(16 * 4 * 4, np.uint8)
+    p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))  # noqa: E731
+    assert goofy_amd.compress_dxt1_fn(p(out), p(img), 12, 4, 48) == -1
+    assert goofy_amd.compress_etc1_fn(p(out), p(img), 16, 3, 64) == -2
+
+
+def test_device_api_validation_without_device():
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.compress_device(goofy_amd.DXT1, 8, 16, 6, 4, 32)
+    assert e.value.status == goofy_amd.ERR_WIDTH
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.compress_device(goofy_amd.DXT1, 8, 16, 8, 6, 32)
+    assert e.value.status == goofy_amd.ERR_HEIGHT
+    for args in [(goofy_amd.DXT1, 8, 16, 8, 4, 24),      # pitch not a multiple of 16
+                 (goofy_amd.DXT1, 8, 16, 8, 4, 16),      # pitch < 4*width
+                 (goofy_amd.DXT1, 8, 20, 8, 4, 32),      # misaligned input
+                 (goofy_amd.DXT1, 0, 16, 8, 4, 32),      # null output
+                 (7, 8, 16, 8, 4, 32)]:                  # unknown format
+        with pytest.raises(goofy_amd.GoofyError) as e:
+            goofy_amd.compress_device(*args)
+        assert e.value.status == goofy_amd.ERR_INVALID, args
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.compress_device(goofy_amd.DXT1, 8, 16, 4, 4, 16, config={"blocks_per_lane": 3, "wg_waves": 4,
+                                                                            "waves_per_cu": 0, "nontemporal": 0})
+    assert e.value.status == goofy_amd.ERR_INVALID
+
+
+def test_batch_validation_without_device():
+    bad = [dict(src=16, dst=8, width=6, height=4, pitch=32, format=0)]
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.Batch(bad)
+    assert e.value.status == goofy_amd.ERR_WIDTH
+    bad = [dict(src=16, dst=8, width=8, height=4, pitch=32, format=0),
+           dict(src=16, dst=8, width=8, height=4, pitch=40, format=0)]
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.Batch(bad)
+    assert e.value.status == goofy_amd.ERR_INVALID and "texture 1" in goofy_amd.last_error()
+
+
+def test_no_device_is_reported():
+    if goofy_amd.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.compress(goofy_amd.DXT1, np.zeros(16 * 4 * 4, np.uint8), 16, 4)
+    assert e.value.status == goofy_amd.ERR_NO_DEVICE
+
+
+def test_synth_host_deterministic_and_threaded():
+    a = goofy_amd.synth_host(0, 77, 3, 20, 100, 52, threads=1)
+    b = goofy_amd.synth_host(0, 77, 3, 20, 100, 52, threads=7)
+    assert np.array_equal(a, b)
+    c = goofy_amd.synth_host(1, 77, 3, 0, 130, 70, pitch=130 * 4 + 24, fill=0xEE)
+    rows = c.reshape(70, 130 * 4 + 24)
+    assert (rows[:, 130 * 4:] == 0xEE).all()          # padding untouched
+    assert (rows[:, 3:130 * 4:4] == 255).all()          # alpha = 255

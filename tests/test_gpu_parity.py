@@ -1,0 +1,259 @@
+"""Parity of the HIP encoder with the reference, on the GPU, through the C-ABI.
+
+Bit-exact on every block: against the golden vectors the reference wrote
+(tests/golden/), and against the CPU oracle on the same seeded inputs.
+"""
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+from goofy_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev_buffer(nbytes, align=256, fill=None):
+    import torch
+    raw = torch.empty(nbytes + align, dtype=torch.uint8, device="cuda")
+    off = (-raw.data_ptr()) % align
+    t = raw[off:off + nbytes]
+    if fill is not None:
+        t.fill_(fill)
+    return t
+
+
+def to_host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def upload(a, align=256):
+    import torch
+    t = dev_buffer(a.nbytes, align)
+    t.copy_(torch.from_numpy(np.ascontiguousarray(a).reshape(-1)))
+    return t
+
+
+def encode_dev(g, fmt, src, w, h, pitch, config=None):
+    out = dev_buffer(max(8, (w // 4) * (h // 4) * 8), fill=0xCD)
+    g.compress_device(fmt, out, src, w, h, pitch, config=config)
+    return to_host(out)[: (w // 4) * (h // 4) * 8]
+
+
+# ---- generator ----------------------------------------------------------------------
+
+@pytest.mark.parametrize("kind,w,h,pitch", [(0, 512, 512, 2048), (1, 300, 132, 1280), (0, 1028, 36, 4112)])
+def test_synth_device_matches_host(gpu, kind, w, h, pitch):
+    say(f"synth kind={kind} {w}x{h} pitch={pitch}")
+    host = gpu.synth_host(kind, 99, 4, 17, w, h, pitch, fill=0x5A)
+    dev = dev_buffer(pitch * h, fill=0x5A)
+    gpu.synth_device(kind, 99, 4, 17, dev, w, h, pitch)
+    assert np.array_equal(to_host(dev), host)
+
+
+# ---- drop-in (host memory) ------------------------------------------------------------
+
+def test_dropin_img512_golden(gpu, golden_img512, digests):
+    im = W.IMAGE_512
+    img = gpu.synth_host(im.kind, im.seed, im.texture_id, im.noise, im.width, im.height, im.pitch)
+    for fmt, key, fn in ((W.DXT1, "dxt1", gpu.compressDXT1), (W.ETC1, "etc1", gpu.compressETC1)):
+        say(f"drop-in {key} 512x512")
+        out = np.full(im.blocks * 8, 0xCD, np.uint8)
+        assert fn(out, img, im.width, im.height, im.pitch) == 0
+        assert np.array_equal(out, golden_img512[key])
+
+
+@pytest.mark.parametrize("w,h,extra", [(16, 4, 0), (32, 8, 16), (48, 100, 4), (528, 36, 200), (1024, 4, 0)])
+def test_dropin_strides_vs_oracle(gpu, oracle, w, h, extra):
+    rng = np.random.default_rng(w * 7 + h)
+    stride = w * 4 + extra
+    img = rng.integers(0, 256, stride * h, dtype=np.uint8)  # row padding holds garbage
+    for fmt in (W.DXT1, W.ETC1):
+        say(f"drop-in fmt={fmt} {w}x{h} stride={stride}")
+        rc_o, want = oracle.compress(fmt, img, w, h, stride)
+        out = np.full(max(1, (w // 4) * (h // 4) * 8), 0xCD, np.uint8)
+        assert gpu.compress_dxt1_fn is not None
+        fn = gpu.compressDXT1 if fmt == W.DXT1 else gpu.compressETC1
+        assert fn(out, img, w, h, stride) == rc_o == 0
+        assert np.array_equal(out[: want.size], want)
+
+
+def test_dropin_return_codes_on_gpu(gpu):
+    out = np.zeros(4096, np.uint8)
+    img = np.zeros(64 * 64 * 4, np.uint8)
+    assert gpu.compressDXT1(out, img, 60, 64, 240) == -1
+    assert gpu.compressETC1(out, img, 64, 62, 256) == -2
+    assert gpu.compressETC1(out, img, 0, 8, 0) == 0
+
+
+# ---- device memory ------------------------------------------------------------------------
+
+def test_device_golden_blocks(gpu, golden_blocks):
+    blocks = golden_blocks["blocks"]
+    n = blocks.shape[0]
+    img = np.ascontiguousarray(blocks.reshape(n, 4, 16).transpose(1, 0, 2)).reshape(-1)  # 4 rows, n blocks wide
+    src = upload(img)
+    for fmt, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+        say(f"golden corpus {key}: {n} blocks")
+        out = encode_dev(gpu, fmt, src, 4 * n, 4, 16 * n).reshape(n, 8)
+        bad = np.nonzero((out != golden_blocks[key]).any(1))[0]
+        assert bad.size == 0, f"{bad.size} blocks differ, first {bad[:8]}"
+
+
+@pytest.mark.parametrize("name", ["waves-8192x8192", "stress-16384x16384-pitch65792"])
+def test_device_large_images_golden(gpu, digests, name):
+    im = {W.IMAGE_8192.name: W.IMAGE_8192, W.STRESS_16384.name: W.STRESS_16384}[name]
+    d = digests["images"][name]
+    say(f"device {name}: synth on device")
+    src = dev_buffer(im.pitch * im.height, fill=0)
+    gpu.synth_device(im.kind, im.seed, im.texture_id, im.noise, src, im.width, im.height, im.pitch)
+    assert sha(to_host(src)) == d["input_sha256"]
+    for fmt, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+        say(f"device {name} {key}")
+        out = encode_dev(gpu, fmt, src, im.width, im.height, im.pitch)
+        assert sha(out) == d[f"{key}_sha256"], key
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_device_shapes_vs_oracle(gpu, oracle, seed):
+    # widths that are multiples of 4 (not 16), padded pitches with poison in
+    # the padding, input offsets that are 16- but not 256-byte aligned
+    rng = np.random.default_rng(seed)
+    w = int(rng.integers(1, 300)) * 4
+    h = int(rng.integers(1, 60)) * 4
+    pitch = w * 4 + int(rng.integers(0, 5)) * 16
+    img = rng.integers(0, 256, pitch * h, dtype=np.uint8)
+    if seed % 2:
+        img = np.clip(img // 16 + rng.integers(0, 230), 0, 255).astype(np.uint8)
+    raw = upload(np.concatenate([np.zeros(48, np.uint8), img]))
+    src = raw[48:]
+    for fmt in (W.DXT1, W.ETC1):
+        say(f"shape seed={seed} {w}x{h} pitch={pitch} fmt={fmt}")
+        want = oracle.compress_blocks(fmt, img, w, h, pitch)
+        assert np.array_equal(encode_dev(gpu, fmt, src, w, h, pitch), want)
+
+
+@pytest.mark.parametrize("bpl", [1, 2, 4])
+@pytest.mark.parametrize("wg", [1, 2, 4, 8])
+@pytest.mark.parametrize("nt", [0, 1])
+def test_launch_configs_vs_oracle(gpu, oracle, bpl, wg, nt):
+    w, h, pitch = 1028, 260, 1028 * 4 + 48
+    img = gpu.synth_host(0, 1234, 0, 25, w, h, pitch, fill=0x77)
+    src = upload(img)
+    cfg = dict(blocks_per_lane=bpl, wg_waves=wg, waves_per_cu=0, nontemporal=nt)
+    for fmt in (W.DXT1, W.ETC1):
+        say(f"config {cfg} fmt={fmt}")
+        assert np.array_equal(encode_dev(gpu, fmt, src, w, h, pitch, cfg), oracle.compress_blocks(fmt, img, w, h, pitch))
+
+
+# ---- batched --------------------------------------------------------------------------------
+
+def make_batch_inputs(g, textures):
+    items = []
+    for t in textures:
+        src = dev_buffer(t["width"] * 4 * t["height"])
+        g.synth_device(t["kind"], t["seed"], t["texture_id"], t["noise"], src, t["width"], t["height"])
+        dst = dev_buffer((t["width"] // 4) * (t["height"] // 4) * 8, fill=0xCD)
+        items.append(dict(src=src, dst=dst, width=t["width"], height=t["height"], pitch=t["width"] * 4,
+                          format=t["format"], meta=t))
+    return items
+
+
+def test_batch_golden16(gpu, digests):
+    ts = digests["batch"]
+    items = make_batch_inputs(gpu, ts)
+    say(f"batch golden: {len(items)} textures")
+    b = gpu.Batch(items)
+    b.launch()
+    for it, t in zip(items, ts):
+        assert sha(to_host(it["src"])) == t["input_sha256"]
+        assert sha(to_host(it["dst"])) == t["output_sha256"], t
+
+
+def test_batch_vs_oracle_and_configs(gpu, oracle):
+    import torch
+    ts = W.batch_textures(n=40, lo=4, hi=700, seed=77)
+    items = make_batch_inputs(gpu, ts)
+    want = []
+    for it in items:
+        img = to_host(it["src"])
+        want.append(oracle.compress_blocks(it["format"], img, it["width"], it["height"], it["pitch"]))
+    b = gpu.Batch(items)
+    say(f"batch 40 textures, info={b.info()}, config={b.config}")
+
+    def check(tag):
+        for it, wv in zip(items, want):
+            assert np.array_equal(to_host(it["dst"]), wv), (tag, it["meta"])
+
+    def clear():
+        for it in items:
+            it["dst"].fill_(0xCD)
+
+    b.launch()
+    check("graph")
+    # replays of the captured graph, with eager device work in between
+    for r in range(3):
+        clear()
+        torch.cuda.synchronize()
+        x = torch.arange(1 << 20, device="cuda").sum()
+        b.launch()
+        torch.cuda.synchronize()
+        assert int(x) > 0
+        check(f"replay {r}")
+    clear()
+    b.launch_per_texture()
+    check("per-texture")
+    for cfg in (dict(blocks_per_lane=1, wg_waves=2, waves_per_cu=0, nontemporal=0),
+                dict(blocks_per_lane=4, wg_waves=8, waves_per_cu=1, nontemporal=1),
+                dict(blocks_per_lane=2, wg_waves=1, waves_per_cu=3, nontemporal=0)):
+        clear()
+        b.config = cfg
+        say(f"batch config {cfg}")
+        b.launch()
+        check(str(cfg))
+    clear()
+    chosen, ms = b.autotune(reps=1)
+    say(f"autotune chose {chosen} ({ms:.3f} ms)")
+    clear()
+    b.launch()
+    check("autotuned")
+    b.close()
+
+
+def test_batch_full_workload(gpu, oracle):
+    """The 4096-texture workload: batch output equals per-texture launches on
+    every texture, and the oracle on a sample of textures."""
+    import torch
+    ts = W.batch_textures()
+    say("batch 4096: generating on device")
+    items = make_batch_inputs(gpu, ts)
+    b = gpu.Batch(items)
+    say(f"batch 4096 info={b.info()}")
+    b.launch()
+    torch.cuda.synchronize()
+    first = [it["dst"].clone() for it in items]
+    for it in items:
+        it["dst"].fill_(0xCD)
+    b.launch_per_texture()
+    torch.cuda.synchronize()
+    for f, it in zip(first, items):
+        assert torch.equal(f, it["dst"]), it["meta"]
+    rng = np.random.default_rng(0)
+    for i in rng.choice(len(items), 48, replace=False):
+        it = items[int(i)]
+        img = to_host(it["src"])
+        want = oracle.compress_blocks(it["format"], img, it["width"], it["height"], it["pitch"])
+        assert np.array_equal(to_host(first[int(i)]), want), it["meta"]
+    b.close()
