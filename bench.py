@@ -40,36 +40,30 @@ def sha(a) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def aligned_dev(torch, nbytes: int, align: int = 256):
-    raw = torch.empty(nbytes + align, dtype=torch.uint8, device="cuda")
-    off = (-raw.data_ptr()) % align
-    return raw[off:off + nbytes]
-
-
 # ---- plain 8192^2 ------------------------------------------------------------------------
 
-def bench_plain(g, torch, W, steps: int, warmup: int, rotate: int, cfg: dict | None):
+def bench_plain(g, hip, W, steps: int, warmup: int, rotate: int, cfg: dict | None):
     im = W.IMAGE_8192
     nbytes = im.pitch * im.height
     out_bytes = im.blocks * 8
     log(f"plain: generating {rotate} x {im.name} on device")
     srcs = []
+    stream = hip.Stream()
     for _ in range(rotate):
-        s = aligned_dev(torch, nbytes)
-        g.synth_device(im.kind, im.seed, im.texture_id, im.noise, s, im.width, im.height, im.pitch)
+        s = hip.DeviceBuffer(nbytes)
+        g.synth_device(im.kind, im.seed, im.texture_id, im.noise, s, im.width, im.height, im.pitch, stream)
         srcs.append(s)
-    outs = {f: [aligned_dev(torch, out_bytes) for _ in range(rotate)] for f in (W.DXT1, W.ETC1)}
-    stream = torch.cuda.current_stream()
-    torch.cuda.synchronize()
+    outs = {f: [hip.DeviceBuffer(out_bytes) for _ in range(rotate)] for f in (W.DXT1, W.ETC1)}
+    stream.synchronize()
 
     # parity of the measured path against the reference's digests
     digests = json.loads((ROOT / "tests" / "golden" / "digests.json").read_text())["images"][im.name]
     parity = {}
     for f, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
         g.compress_device(f, outs[f][0], srcs[0], im.width, im.height, im.pitch, stream, cfg)
-        torch.cuda.synchronize()
-        parity[key] = sha(outs[f][0].cpu().numpy()) == digests[f"{key}_sha256"]
-    parity["input"] = sha(srcs[0].cpu().numpy()) == digests["input_sha256"]
+        stream.synchronize()
+        parity[key] = sha(outs[f][0].download()) == digests[f"{key}_sha256"]
+    parity["input"] = sha(srcs[0].download()) == digests["input_sha256"]
 
     j = 0
 
@@ -78,7 +72,7 @@ def bench_plain(g, torch, W, steps: int, warmup: int, rotate: int, cfg: dict | N
         for f in (W.DXT1, W.ETC1):
             k = j % rotate
             if events is not None:
-                events[f].append((torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)))
+                events[f].append((hip.Event(), hip.Event()))
                 events[f][-1][0].record(stream)
             g.compress_device(f, outs[f][k], srcs[k], im.width, im.height, im.pitch, stream, cfg)
             if events is not None:
@@ -88,44 +82,46 @@ def bench_plain(g, torch, W, steps: int, warmup: int, rotate: int, cfg: dict | N
     log(f"plain: warmup {warmup} steps")
     for _ in range(warmup):
         step()
-    torch.cuda.synchronize()
+    stream.synchronize()
     # timed region: wall clock around exactly `steps` steps, events per launch
     events = {W.DXT1: [], W.ETC1: []}
     log(f"plain: timing {steps} steps")
     t0 = time.perf_counter()
     for _ in range(steps):
         step(events)
-    torch.cuda.synchronize()
+    stream.synchronize()
     elapsed = time.perf_counter() - t0
-    kern_ms = {f: float(np.mean([a.elapsed_time(b) for a, b in events[f]])) for f in events}
-    kern_med = {f: float(np.median([a.elapsed_time(b) for a, b in events[f]])) for f in events}
-    del srcs, outs
-    torch.cuda.empty_cache()
+    kern_ms = {f: float(np.mean([a.elapsed_ms(b) for a, b in events[f]])) for f in events}
+    kern_med = {f: float(np.median([a.elapsed_ms(b) for a, b in events[f]])) for f in events}
+    for b in srcs + outs[W.DXT1] + outs[W.ETC1]:
+        b.free()
+    stream.destroy()
     return im, elapsed, kern_ms, kern_med, parity
 
 
 # ---- batched --------------------------------------------------------------------------------
 
-def bench_batch(g, torch, W, reps: int = 7):
+def bench_batch(g, hip, W, reps: int = 7):
     ts = W.batch_textures()
     sizes = [t["width"] * 4 * t["height"] for t in ts]
     offs = np.concatenate([[0], np.cumsum([(s + 255) // 256 * 256 for s in sizes])])
     osz = [(t["width"] // 4) * (t["height"] // 4) * 8 for t in ts]
     ooffs = np.concatenate([[0], np.cumsum([(s + 255) // 256 * 256 for s in osz])])
     log(f"batch: {len(ts)} textures, {offs[-1] / 2**30:.2f} GiB in, generating on device")
-    src = aligned_dev(torch, int(offs[-1]))
-    dst = aligned_dev(torch, int(ooffs[-1]))
-    dst2 = aligned_dev(torch, int(ooffs[-1]))
+    stream = hip.Stream()
+    src = hip.DeviceBuffer(int(offs[-1]))
+    dst = hip.DeviceBuffer(int(ooffs[-1]), fill=0)
+    dst2 = hip.DeviceBuffer(int(ooffs[-1]), fill=0)
     base, obase, obase2 = src.data_ptr(), dst.data_ptr(), dst2.data_ptr()
     items, items2 = [], []
     for i, t in enumerate(ts):
-        g.synth_device(t["kind"], t["seed"], t["texture_id"], t["noise"], base + int(offs[i]), t["width"], t["height"])
+        g.synth_device(t["kind"], t["seed"], t["texture_id"], t["noise"], base + int(offs[i]), t["width"], t["height"],
+                       stream=stream)
         d = dict(src=base + int(offs[i]), width=t["width"], height=t["height"], pitch=t["width"] * 4,
                  format=t["format"])
         items.append({**d, "dst": obase + int(ooffs[i])})
         items2.append({**d, "dst": obase2 + int(ooffs[i])})
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
+    stream.synchronize()
     b = g.Batch(items)
     b2 = g.Batch(items2)
     info = b.info()
@@ -135,22 +131,22 @@ def bench_batch(g, torch, W, reps: int = 7):
     def timed(fn, n):
         ts_ = []
         for _ in range(n):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0, e1 = hip.Event(), hip.Event()
             e0.record(stream)
             fn(stream)
             e1.record(stream)
             e1.synchronize()
-            ts_.append(e0.elapsed_time(e1))
+            ts_.append(e0.elapsed_ms(e1))
         return float(np.median(ts_))
 
     b.launch(stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     graph_ms = timed(b.launch, reps)
     b2.launch_per_texture(stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     plain_ms = timed(b2.launch_per_texture, max(3, reps // 2))
-    torch.cuda.synchronize()
-    same = bool(torch.equal(dst, dst2))
+    stream.synchronize()
+    same = bool(np.array_equal(dst.download(), dst2.download()))
     px = info["pixels"]
     res = {
         "workload": "4096 textures, W,H log-uniform multiples of 4 in [64,2048], 50/50 DXT1/ETC1S, noise U[0,40]",
@@ -163,8 +159,9 @@ def bench_batch(g, torch, W, reps: int = 7):
     }
     b.close()
     b2.close()
-    del src, dst, dst2
-    torch.cuda.empty_cache()
+    for x in (src, dst, dst2):
+        x.free()
+    stream.destroy()
     return res
 
 
@@ -230,22 +227,24 @@ def load_traffic():
         return None
 
 
-def profile_mode(g, torch, W, launches: int):
+def profile_mode(g, hip, W, launches: int):
     """Short run for rocprofv3: each plain kernel `launches` times, plus a
-    plain 256 MiB int32 copy kernel used to calibrate the HBM byte counters."""
+    device-to-device copy of the 256 MiB image (the runtime's copy kernel)
+    used to calibrate the HBM byte counters."""
     im = W.IMAGE_8192
-    src = aligned_dev(torch, im.pitch * im.height)
-    g.synth_device(im.kind, im.seed, im.texture_id, im.noise, src, im.width, im.height, im.pitch)
-    out = aligned_dev(torch, im.blocks * 8)
-    a = src.view(torch.int32)
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
+    nbytes = im.pitch * im.height
+    srcs = [hip.DeviceBuffer(nbytes) for _ in range(4)]
+    for s in srcs:
+        g.synth_device(im.kind, im.seed, im.texture_id, im.noise, s, im.width, im.height, im.pitch)
+    out = hip.DeviceBuffer(im.blocks * 8)
+    copy_dst = hip.DeviceBuffer(nbytes)
+    hip.synchronize()
     for i in range(launches):
-        torch.bitwise_or(a, 0, out=b)  # calibration: reads and writes 256 MiB
+        copy_dst.copy_from(srcs[i % 4])  # calibration: reads and writes 256 MiB
     for f in (W.DXT1, W.ETC1):
         for i in range(launches):
-            g.compress_device(f, out, src, im.width, im.height, im.pitch)
-    torch.cuda.synchronize()
+            g.compress_device(f, out, srcs[i % 4], im.width, im.height, im.pitch)
+    hip.synchronize()
     log(f"profile mode done: {launches} launches per kernel")
 
 
@@ -269,15 +268,13 @@ def main() -> int:
               "configuration is not part of this build (run with --gpus 1)", file=sys.stderr)
         return 2
 
-    import torch
-
     import goofy_amd as g
+    from goofy_amd import hip
     from goofy_amd import workloads as W
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    torch.cuda.set_device(0)
+    assert hip.device_count() > 0, "bench.py needs a GPU"
     if args.profile:
-        profile_mode(g, torch, W, args.profile)
+        profile_mode(g, hip, W, args.profile)
         return 0
 
     cfg = None
@@ -285,7 +282,7 @@ def main() -> int:
         cfg = dict(blocks_per_lane=args.bpl or 2, wg_waves=args.wg or 4, waves_per_cu=0,
                    nontemporal=max(0, args.nt))
 
-    im, elapsed, kern_ms, kern_med, parity = bench_plain(g, torch, W, args.steps, args.warmup, args.rotate, cfg)
+    im, elapsed, kern_ms, kern_med, parity = bench_plain(g, hip, W, args.steps, args.warmup, args.rotate, cfg)
     ms_per_step = elapsed / args.steps * 1e3
     value = 2 * im.pixels / 1e6 / (ms_per_step / 1e3)
     algo = W.algorithmic_bytes(im.width, im.height)
@@ -312,10 +309,10 @@ def main() -> int:
         "roofline": roofline, "formats": formats, "parity": parity,
     }
     if not args.skip_batch:
-        result["batched"] = bench_batch(g, torch, W)
+        result["batched"] = bench_batch(g, hip, W)
     if not args.skip_serve and hasattr(g, "serve_simulate"):
         from goofy_amd import serving
-        result["served"] = serving.bench_serving(g, torch, W, log=log)
+        result["served"] = serving.bench_serving(g, hip, W, log=log)
     if not args.skip_cpu:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(g, W)

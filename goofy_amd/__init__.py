@@ -132,7 +132,7 @@ def _u8(a) -> ctypes.POINTER(ctypes.c_uint8):
 
 
 def _ptr(x) -> int | None:
-    """Device address from an int or a torch tensor (data_ptr)."""
+    """Device address from an int or any object with data_ptr() (goofy_amd.hip.DeviceBuffer)."""
     if x is None:
         return None
     if isinstance(x, int):
@@ -141,9 +141,10 @@ def _ptr(x) -> int | None:
 
 
 def _stream(s) -> int | None:
+    """Stream handle from None (default stream), an int, or a goofy_amd.hip.Stream."""
     if s is None or isinstance(s, int):
         return s
-    return int(s.cuda_stream)  # torch.cuda.Stream
+    return int(s)
 
 
 # ---- drop-in (host memory) ------------------------------------------------------
@@ -170,7 +171,7 @@ def compress(fmt: int, image: np.ndarray, width: int, height: int, stride: int |
 
 def compress_device(fmt: int, dst, src, width: int, height: int, pitch: int, stream=None,
                     config: LaunchConfig | dict | None = None) -> None:
-    """Enqueue the encode of a device-resident image (dst/src: device addresses or torch tensors)."""
+    """Enqueue the encode of a device-resident image (dst/src: device addresses or DeviceBuffers)."""
     if config is None:
         _check(_lib.goofy_hip_compress_device(fmt, _ptr(dst), _ptr(src), width, height, pitch, _stream(stream)))
     else:

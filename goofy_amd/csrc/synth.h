@@ -118,10 +118,15 @@ GOOFY_HD uint32_t pixel_stress(const Params& P, uint32_t x, uint32_t y)
         const int32_t nx = (11469 * isin(mix32(h ^ 6u) + k1 * x + (k2 >> 1) * y)) / 16384;
         const int32_t ny = (11469 * isin(mix32(h ^ 7u) + (k1 >> 2) * x + k2 * y)) / 16384;
         const uint32_t rest = (1u << 28) - (uint32_t)(nx * nx) - (uint32_t)(ny * ny);
-        const int32_t nz = (int32_t)isqrt32(rest);
-        const uint32_t r = clamp255(((nx + 16384) * 255 + 16384) / 32768);
-        const uint32_t g = clamp255(((ny + 16384) * 255 + 16384) / 32768);
-        const uint32_t b = clamp255(((nz + 16384) * 255 + 16384) / 32768);
+        const uint32_t nz = isqrt32(rest);  // <= 16384
+        // n in [-16384, 16384] -> round(255 * (n + 1) / 2): always in [0, 255],
+        // so no clamp.  (A clamp-and-pack of two channels here was turned by
+        // the gfx950 backend into v_ashr_pk_u8_i32 followed by an add that
+        // assumed the instruction's upper 16 result bits were zero; on the
+        // device they were not.  Unsigned, clamp-free arithmetic avoids it.)
+        const uint32_t r = ((uint32_t)(nx + 16384) * 255u + 16384u) >> 15;
+        const uint32_t g = ((uint32_t)(ny + 16384) * 255u + 16384u) >> 15;
+        const uint32_t b = ((nz + 16384u) * 255u + 16384u) >> 15;
         return 0xFF000000u | r | (g << 8) | (b << 16);
     }
     }
