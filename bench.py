@@ -72,11 +72,12 @@ def bench_plain(g, hip, W, steps: int, warmup: int, rotate: int, cfg: dict | Non
         for f in (W.DXT1, W.ETC1):
             k = j % rotate
             if events is not None:
+                # start/stop recorded by the kernel's own dispatch packet
                 events[f].append((hip.Event(), hip.Event()))
-                events[f][-1][0].record(stream)
-            g.compress_device(f, outs[f][k], srcs[k], im.width, im.height, im.pitch, stream, cfg)
-            if events is not None:
-                events[f][-1][1].record(stream)
+                g.compress_device_timed(f, outs[f][k], srcs[k], im.width, im.height, im.pitch,
+                                        events[f][-1][0], events[f][-1][1], stream, cfg)
+            else:
+                g.compress_device(f, outs[f][k], srcs[k], im.width, im.height, im.pitch, stream, cfg)
             j += 1
 
     log(f"plain: warmup {warmup} steps")
@@ -279,8 +280,8 @@ def main() -> int:
 
     cfg = None
     if args.bpl or args.wg or args.nt >= 0:
-        cfg = dict(blocks_per_lane=args.bpl or 2, wg_waves=args.wg or 4, waves_per_cu=0,
-                   nontemporal=max(0, args.nt))
+        cfg = dict(blocks_per_lane=args.bpl or 1, wg_waves=args.wg or 2, waves_per_cu=0,
+                   nontemporal=1 if args.nt < 0 else args.nt)
 
     im, elapsed, kern_ms, kern_med, parity = bench_plain(g, hip, W, args.steps, args.warmup, args.rotate, cfg)
     ms_per_step = elapsed / args.steps * 1e3

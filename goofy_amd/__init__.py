@@ -44,7 +44,7 @@ class GoofyError(RuntimeError):
 
 
 if not _LIB_PATH.exists():
-    raise ImportError(f"{_LIB_PATH} is not built: run `python -m goofy_amd.build` "
+    raise ImportError(f"{_LIB_PATH} is not built: run `python goofy_amd/build.py` "
                       "(the encoder has no CPU fallback)")
 
 _lib = ctypes.CDLL(str(_LIB_PATH))
@@ -86,6 +86,8 @@ _sig("goofy_hip_compress_etc1", ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
 _sig("goofy_hip_compress_device", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp)
 _sig("goofy_hip_compress_device_cfg", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp,
      ctypes.POINTER(LaunchConfig))
+_sig("goofy_hip_compress_device_timed", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp,
+     ctypes.POINTER(LaunchConfig), _vp, _vp)
 _sig("goofy_batch_create", ctypes.c_int, ctypes.POINTER(TextureDesc), _sz, ctypes.POINTER(LaunchConfig),
      ctypes.POINTER(_vp))
 _sig("goofy_batch_launch", ctypes.c_int, _vp, _vp)
@@ -178,6 +180,18 @@ def compress_device(fmt: int, dst, src, width: int, height: int, pitch: int, str
         cfg = config if isinstance(config, LaunchConfig) else LaunchConfig(**config)
         _check(_lib.goofy_hip_compress_device_cfg(fmt, _ptr(dst), _ptr(src), width, height, pitch,
                                                   _stream(stream), ctypes.byref(cfg)))
+
+
+def compress_device_timed(fmt: int, dst, src, width: int, height: int, pitch: int, start_event, stop_event,
+                          stream=None, config: LaunchConfig | dict | None = None) -> None:
+    """compress_device with hipEvents recorded by the kernel's own dispatch (goofy_amd.hip.Event)."""
+    cfg = None
+    if config is not None:
+        cfg = config if isinstance(config, LaunchConfig) else LaunchConfig(**config)
+    _check(_lib.goofy_hip_compress_device_timed(fmt, _ptr(dst), _ptr(src), width, height, pitch, _stream(stream),
+                                                ctypes.byref(cfg) if cfg else None,
+                                                start_event.handle if start_event else None,
+                                                stop_event.handle if stop_event else None))
 
 
 class Batch:

@@ -1,6 +1,7 @@
 """Build libgoofy_hip.so (gfx950) in-tree, and the test-only oracle libraries.
 
-Usage: python -m goofy_amd.build [--no-oracle]
+Usage: python goofy_amd/build.py [--no-oracle]   (run as a script: importing the
+package would load the library being rebuilt)
 """
 from __future__ import annotations
 

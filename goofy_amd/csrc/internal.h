@@ -42,7 +42,7 @@ inline bool magic_ok(uint64_t bw, uint64_t nblocks) { return bw > 0 && nblocks *
 
 // Kernel launchers (kernels.hip).
 hipError_t launch_plain(int fmt, const LaunchCfg& cfg, const void* src, size_t pitch, void* dst, uint32_t bw,
-                        uint32_t nblocks, hipStream_t stream);
+                        uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_batch(const LaunchCfg& cfg, const TexDesc* d_descs, uint32_t ntex, uint32_t ntiles, int num_cus,
                         hipStream_t stream);
 hipError_t launch_synth(const void* params, void* dst, uint32_t width, uint32_t height, size_t pitch,

@@ -11,6 +11,7 @@
 // This replaces the reference driver loop (GoofyTC/goofy_tc.h:1514-1524,
 // 4 blocks per call) and goofySimdEncode (:1069-1494); the arithmetic of a
 // block is in block_codec.h.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "block_codec.h"
@@ -123,32 +124,43 @@ __global__ __launch_bounds__(256) void k_synth(goofy_synth::Params P, uint8_t* _
 
 // ---- launchers ---------------------------------------------------------------
 
-template <int FMT, int BPL>
-static hipError_t plain_nt(bool nt, dim3 grid, dim3 block, const void* src, size_t pitch, void* dst, uint32_t bw,
-                           uint32_t nblocks, uint64_t magic, hipStream_t s)
+// With a start/stop event pair the launch goes through hipExtLaunchKernelGGL,
+// which has the dispatch packet itself record both events: the interval is
+// the kernel's execution, without marker packets around it.
+template <int FMT, int BPL, bool NT>
+static hipError_t plain_go(dim3 grid, dim3 block, const void* src, size_t pitch, void* dst, uint32_t bw,
+                           uint32_t nblocks, uint64_t magic, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
-    if (nt)
-        hipLaunchKernelGGL((k_encode_plain<FMT, BPL, true>), grid, block, 0, s, (const uint8_t*)src, (uint64_t)pitch,
-                           (uint8_t*)dst, bw, nblocks, magic);
+    if (e0 || e1)
+        hipExtLaunchKernelGGL((k_encode_plain<FMT, BPL, NT>), grid, block, 0, s, e0, e1, 0, (const uint8_t*)src,
+                              (uint64_t)pitch, (uint8_t*)dst, bw, nblocks, magic);
     else
-        hipLaunchKernelGGL((k_encode_plain<FMT, BPL, false>), grid, block, 0, s, (const uint8_t*)src,
-                           (uint64_t)pitch, (uint8_t*)dst, bw, nblocks, magic);
+        hipLaunchKernelGGL((k_encode_plain<FMT, BPL, NT>), grid, block, 0, s, (const uint8_t*)src, (uint64_t)pitch,
+                           (uint8_t*)dst, bw, nblocks, magic);
     return hipGetLastError();
 }
 
 template <int FMT>
 static hipError_t plain_bpl(const LaunchCfg& c, dim3 grid, dim3 block, const void* src, size_t pitch, void* dst,
-                            uint32_t bw, uint32_t nblocks, uint64_t magic, hipStream_t s)
+                            uint32_t bw, uint32_t nblocks, uint64_t magic, hipStream_t s, hipEvent_t e0,
+                            hipEvent_t e1)
 {
+    const bool nt = c.nontemporal != 0;
     switch (c.blocks_per_lane) {
-    case 1: return plain_nt<FMT, 1>(c.nontemporal, grid, block, src, pitch, dst, bw, nblocks, magic, s);
-    case 2: return plain_nt<FMT, 2>(c.nontemporal, grid, block, src, pitch, dst, bw, nblocks, magic, s);
-    default: return plain_nt<FMT, 4>(c.nontemporal, grid, block, src, pitch, dst, bw, nblocks, magic, s);
+    case 1:
+        return nt ? plain_go<FMT, 1, true>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1)
+                  : plain_go<FMT, 1, false>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1);
+    case 2:
+        return nt ? plain_go<FMT, 2, true>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1)
+                  : plain_go<FMT, 2, false>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1);
+    default:
+        return nt ? plain_go<FMT, 4, true>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1)
+                  : plain_go<FMT, 4, false>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1);
     }
 }
 
 hipError_t launch_plain(int fmt, const LaunchCfg& c, const void* src, size_t pitch, void* dst, uint32_t bw,
-                        uint32_t nblocks, hipStream_t stream)
+                        uint32_t nblocks, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
 {
     if (nblocks == 0) return hipSuccess;
     const uint32_t tile = 64u * (uint32_t)c.blocks_per_lane;
@@ -156,8 +168,9 @@ hipError_t launch_plain(int fmt, const LaunchCfg& c, const void* src, size_t pit
     const uint32_t wg = (uint32_t)c.wg_waves;
     const dim3 grid((ntiles + wg - 1) / wg), block(64u * wg);
     const uint64_t magic = div_magic(bw);
-    if (fmt == GOOFY_FORMAT_DXT1) return plain_bpl<0>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream);
-    return plain_bpl<1>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream);
+    if (fmt == GOOFY_FORMAT_DXT1)
+        return plain_bpl<0>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream, start, stop);
+    return plain_bpl<1>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream, start, stop);
 }
 
 template <int BPL>

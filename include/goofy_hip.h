@@ -82,6 +82,13 @@ GOOFY_HIP_API int goofy_hip_compress_device_cfg(int format, void* d_result, cons
                                                 unsigned int height, size_t pitch, void* stream,
                                                 const goofy_launch_config* cfg);
 
+/* As goofy_hip_compress_device_cfg, with two hipEvent_t handles that the
+ * kernel's dispatch packet records at its start and end (either may be NULL):
+ * hipEventElapsedTime(start, stop) is then the kernel's own execution time. */
+GOOFY_HIP_API int goofy_hip_compress_device_timed(int format, void* d_result, const void* d_input,
+                                                  unsigned int width, unsigned int height, size_t pitch, void* stream,
+                                                  const goofy_launch_config* cfg, void* start_event, void* stop_event);
+
 /* ---- batched encode: many textures, one launch -----------------------------
  * A plan flattens the block work-lists of n device-resident textures (each
  * its own size, pitch and format) into one wave-tile list and encodes all of
