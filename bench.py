@@ -258,6 +258,7 @@ def main() -> int:
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--profile", type=int, default=0, help="rocprof mode: N launches per kernel, no JSON")
     ap.add_argument("--bpl", type=int, default=0)
     ap.add_argument("--wg", type=int, default=0)
@@ -311,9 +312,9 @@ def main() -> int:
     }
     if not args.skip_batch:
         result["batched"] = bench_batch(g, hip, W)
-    if not args.skip_serve and hasattr(g, "serve_simulate"):
+    if not args.skip_serve:
         from goofy_amd import serving
-        result["served"] = serving.bench_serving(g, hip, W, log=log)
+        result["served"] = serving.bench_serving(g, hip, W, log=log, duration_s=args.serve_seconds)
     if not args.skip_cpu:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(g, W)

@@ -124,6 +124,96 @@ GOOFY_HIP_API int goofy_batch_info(const goofy_batch* batch, uint64_t* textures,
                                    uint64_t* pixels);
 GOOFY_HIP_API int goofy_batch_destroy(goofy_batch* batch);
 
+/* ---- serving: continuous batching under an HBM budget ------------------------
+ * A server owns a scheduler thread, `streams` HIP streams and an output arena
+ * in HBM.  Clients submit requests that point at device-resident RGBA8
+ * textures; the scheduler admits pending requests in arrival order while the
+ * in-flight byte budget (input + output bytes of admitted, unreleased
+ * requests) and the arena allow, packs every group of newly admitted requests
+ * into one batch-kernel launch on a free stream, and keeps launching while
+ * earlier groups are still running, so work of different arrivals overlaps
+ * on the device.  A request's blocks stay in the arena until released. */
+typedef struct goofy_server_config {
+    size_t hbm_budget_bytes;    /* in-flight budget (0 = 32 GiB) */
+    size_t arena_bytes;         /* output arena size (0 = budget / 4) */
+    int streams;                /* concurrent launches in flight (0 = 4) */
+    int max_batch_textures;     /* textures per launch (0 = 256) */
+    int max_pending;            /* queued, not yet admitted requests (0 = 65536) */
+    int batching;               /* 1 = one launch per admitted group, 0 = one plain launch per request */
+    goofy_launch_config launch; /* kernel shape (all zero = library default) */
+} goofy_server_config;
+
+typedef struct goofy_request {
+    const void* src;     /* device RGBA8, 16-byte aligned */
+    unsigned int width;  /* multiple of 4 */
+    unsigned int height; /* multiple of 4 */
+    size_t pitch;        /* >= 4*width, multiple of 16 */
+    int format;          /* goofy_format */
+    uint64_t user_tag;
+} goofy_request;
+
+typedef struct goofy_completion {
+    uint64_t ticket;
+    uint64_t user_tag;
+    int status;          /* GOOFY_OK or an error code */
+    void* dst;           /* device address of the blocks (valid until released) */
+    size_t dst_bytes;
+    double t_submit_us;  /* server clock, microseconds */
+    double t_launch_us;
+    double t_done_us;    /* completion observed by the scheduler */
+} goofy_completion;
+
+typedef struct goofy_server_stats {
+    uint64_t submitted, admitted, completed, released, rejected;
+    uint64_t launches, pending, inflight_requests;
+    uint64_t inflight_bytes, peak_inflight_bytes, budget_bytes;
+} goofy_server_stats;
+
+typedef struct goofy_server goofy_server;
+
+GOOFY_HIP_API int goofy_server_create(const goofy_server_config* cfg, goofy_server** out);
+/* Thread-safe.  GOOFY_ERR_BUSY when the pending queue is full, GOOFY_ERR_INVALID
+ * for a malformed request or one larger than the budget or the arena. */
+GOOFY_HIP_API int goofy_server_submit(goofy_server* server, const goofy_request* request, uint64_t* ticket);
+/* Up to `max` completions, oldest first; *count receives how many. */
+GOOFY_HIP_API int goofy_server_poll(goofy_server* server, goofy_completion* out, int max, int* count);
+/* Frees the request's output range and its budget share. */
+GOOFY_HIP_API int goofy_server_release(goofy_server* server, uint64_t ticket);
+/* Waits until every submitted request has completed (timeout_ms < 0: forever). */
+GOOFY_HIP_API int goofy_server_wait_idle(goofy_server* server, int timeout_ms);
+GOOFY_HIP_API int goofy_server_stats_get(goofy_server* server, goofy_server_stats* stats);
+GOOFY_HIP_API double goofy_server_now_us(const goofy_server* server);
+GOOFY_HIP_API int goofy_server_destroy(goofy_server* server);
+
+/* Load generator around a server: Poisson arrivals at `rate_per_s` (0 =
+ * closed loop: keep `closed_loop_depth` requests outstanding) for
+ * `duration_s`, each request one texture drawn uniformly from `pool`
+ * (device-resident; dst fields ignored).  Completed outputs are released at
+ * once; every `verify_every`-th is first copied back and compared with
+ * `pool_expected[i]` (host bytes of that texture encoded alone; NULL = no check). */
+typedef struct goofy_serve_sim_config {
+    double rate_per_s;
+    double duration_s;
+    uint64_t seed;
+    int closed_loop_depth;
+    const goofy_texture_desc* pool;
+    int pool_count;
+    const unsigned char* const* pool_expected;
+    int verify_every;
+    goofy_server_config server;
+} goofy_serve_sim_config;
+
+typedef struct goofy_serve_sim_result {
+    uint64_t submitted, completed, rejected, verified, mismatches, launches;
+    double wall_s, megapixels, mps, offered_mps;
+    double lat_mean_us, lat_p50_us, lat_p90_us, lat_p99_us, lat_max_us;
+    double queue_p50_us, queue_p99_us; /* submit -> launch */
+    double mean_textures_per_launch;
+    uint64_t peak_inflight_bytes;
+} goofy_serve_sim_result;
+
+GOOFY_HIP_API int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_result* result);
+
 /* ---- synthetic inputs ------------------------------------------------------
  * Seeded, integer-only texture generator (host and device give identical
  * bytes).  kind 0: three 2-D sinusoids per channel + uniform noise in
