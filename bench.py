@@ -94,6 +94,14 @@ def bench_plain(g, hip, W, steps: int, warmup: int, rotate: int, cfg: dict | Non
     elapsed = time.perf_counter() - t0
     kern_ms = {f: float(np.mean([a.elapsed_ms(b) for a, b in events[f]])) for f in events}
     kern_med = {f: float(np.median([a.elapsed_ms(b) for a, b in events[f]])) for f in events}
+    # the same traffic without the encode arithmetic: the attainable floor
+    probe = []
+    for i in range(2 * steps):
+        a, b = hip.Event(), hip.Event()
+        g.traffic_probe(outs[W.DXT1][i % rotate], srcs[i % rotate], im.width, im.height, im.pitch, a, b, stream)
+        probe.append((a, b))
+    stream.synchronize()
+    kern_ms["probe"] = float(np.mean([a.elapsed_ms(b) for a, b in probe[steps // 2:]]))
     for b in srcs + outs[W.DXT1] + outs[W.ETC1]:
         b.free()
     stream.destroy()
@@ -249,6 +257,35 @@ def profile_mode(g, hip, W, launches: int):
     log(f"profile mode done: {launches} launches per kernel")
 
 
+def sweep_mode(g, hip, W, launches: int = 40):
+    """Launch-shape sweep of the plain kernels on the 8192^2 image: median kernel
+    time (dispatch-recorded events) per blocks/lane, waves/workgroup, streaming."""
+    im = W.IMAGE_8192
+    srcs = [hip.DeviceBuffer(im.pitch * im.height) for _ in range(4)]
+    for s in srcs:
+        g.synth_device(im.kind, im.seed, im.texture_id, im.noise, s, im.width, im.height, im.pitch)
+    out = hip.DeviceBuffer(im.blocks * 8)
+    hip.synchronize()
+    algo = W.algorithmic_bytes(im.width, im.height)
+    print("| format | blocks/lane | waves/WG | streaming | median us | GB/s |")
+    print("|---|---|---|---|---|---|")
+    for fmt in (W.DXT1, W.ETC1):
+        for bpl in (1, 2, 4):
+            for wg in (1, 2, 4, 8):
+                for nt in (0, 1):
+                    cfg = dict(blocks_per_lane=bpl, wg_waves=wg, waves_per_cu=0, nontemporal=nt)
+                    for i in range(6):
+                        g.compress_device(fmt, out, srcs[i % 4], im.width, im.height, im.pitch, config=cfg)
+                    ev = [(hip.Event(), hip.Event()) for _ in range(launches)]
+                    for i, (a, b) in enumerate(ev):
+                        g.compress_device_timed(fmt, out, srcs[i % 4], im.width, im.height, im.pitch, a, b,
+                                                config=cfg)
+                    hip.synchronize()
+                    med = float(np.median([a.elapsed_ms(b) for a, b in ev]))
+                    print(f"| {'DXT1' if fmt == W.DXT1 else 'ETC1S'} | {bpl} | {wg} | {nt} | {med * 1e3:.2f} | "
+                          f"{algo / med / 1e6:.0f} |", flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +297,7 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--profile", type=int, default=0, help="rocprof mode: N launches per kernel, no JSON")
+    ap.add_argument("--sweep", action="store_true", help="launch-shape sweep table, no JSON")
     ap.add_argument("--bpl", type=int, default=0)
     ap.add_argument("--wg", type=int, default=0)
     ap.add_argument("--nt", type=int, default=-1)
@@ -278,6 +316,9 @@ def main() -> int:
     if args.profile:
         profile_mode(g, hip, W, args.profile)
         return 0
+    if args.sweep:
+        sweep_mode(g, hip, W)
+        return 0
 
     cfg = None
     if args.bpl or args.wg or args.nt >= 0:
@@ -288,14 +329,16 @@ def main() -> int:
     ms_per_step = elapsed / args.steps * 1e3
     value = 2 * im.pixels / 1e6 / (ms_per_step / 1e3)
     algo = W.algorithmic_bytes(im.width, im.height)
-    dom = max(kern_ms, key=kern_ms.get)
+    dom = max((W.DXT1, W.ETC1), key=kern_ms.get)
     achieved = algo / (kern_ms[dom] / 1e3) / 1e9
+    attainable = algo / (kern_ms["probe"] / 1e3) / 1e9
     traffic = load_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": (traffic or {}).get("bytes_per_launch", {}).get("dxt1" if dom == W.DXT1 else "etc1"),
                 "kernel": "encode ETC1S" if dom == W.ETC1 else "encode DXT1",
-                "algorithmic_bytes_per_launch": algo}
+                "algorithmic_bytes_per_launch": algo,
+                "attainable_gbps": round(attainable, 1), "frac_of_attainable": round(achieved / attainable, 4)}
     formats = {}
     for f, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
         formats[key] = {"kernel_ms_mean": round(kern_ms[f], 5), "kernel_ms_median": round(kern_med[f], 5),

@@ -194,6 +194,52 @@ def compress_device_timed(fmt: int, dst, src, width: int, height: int, pitch: in
                                                 stop_event.handle if stop_event else None))
 
 
+class Quality(ctypes.Structure):
+    _fields_ = ([("pixels", ctypes.c_uint64), ("sse_byte", ctypes.c_uint64 * 4), ("sse_luma", ctypes.c_double)] +
+                [(n, ctypes.c_double) for n in ("mse_r", "mse_g", "mse_b", "mse_a", "mse_max", "mse_rgb", "mse_y",
+                                                "psnr_r", "psnr_g", "psnr_b", "psnr_a", "psnr_min", "psnr_rgb",
+                                                "psnr_y")])
+
+    def as_dict(self) -> dict:
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["sse_byte"] = list(self.sse_byte)
+        return d
+
+
+_sig("goofy_hip_decode_device", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp)
+_sig("goofy_hip_quality_device", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp,
+     ctypes.POINTER(Quality))
+
+
+def decode_device(fmt: int, dst_rgba, blocks, width: int, height: int, pitch: int | None = None,
+                  stream=None) -> None:
+    """Decode device-resident blocks into RGBA8 rows of `pitch` bytes (HIP kernel)."""
+    pitch = width * 4 if pitch is None else pitch
+    _check(_lib.goofy_hip_decode_device(fmt, _ptr(dst_rgba), _ptr(blocks), width, height, pitch, _stream(stream)))
+
+
+def quality_device(fmt: int, blocks, source, width: int, height: int, pitch: int | None = None,
+                   stream=None) -> dict:
+    """Round-trip MSE/PSNR of encoded blocks against the source image (definitions of
+    the reference harness's getMsePsnr, Src/main.cpp:403-469)."""
+    pitch = width * 4 if pitch is None else pitch
+    q = Quality()
+    _check(_lib.goofy_hip_quality_device(fmt, _ptr(blocks), _ptr(source), width, height, pitch, _stream(stream),
+                                         ctypes.byref(q)))
+    return q.as_dict()
+
+
+_sig("goofy_hip_traffic_probe", ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp, _vp, _vp)
+
+
+def traffic_probe(dst, src, width: int, height: int, pitch: int, start_event=None, stop_event=None,
+                  stream=None) -> None:
+    """The encode kernel's memory traffic without its arithmetic (roofline calibration)."""
+    _check(_lib.goofy_hip_traffic_probe(_ptr(dst), _ptr(src), width, height, pitch, _stream(stream),
+                                        start_event.handle if start_event else None,
+                                        stop_event.handle if stop_event else None))
+
+
 class Batch:
     """Many device-resident textures encoded by one persistent-kernel launch (hipGraph replay)."""
 
@@ -280,4 +326,5 @@ __all__ = [
     "DXT1", "ETC1", "FORMATS", "GoofyError", "LaunchConfig", "TextureDesc", "CompressFunc", "Batch",
     "compressDXT1", "compressETC1", "compress", "compress_device", "compress_dxt1_fn", "compress_etc1_fn",
     "synth_host", "synth_device", "last_error", "version", "device_count", "WAVES", "STRESS",
+    "decode_device", "quality_device", "compress_device_timed",
 ]

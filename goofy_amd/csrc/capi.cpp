@@ -4,6 +4,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -289,6 +291,87 @@ int goofy_hip_compress_device_timed(int format, void* d_result, const void* d_in
                                     void* start_event, void* stop_event)
 {
     return compress_device(format, d_result, d_input, width, height, pitch, stream, cfg, start_event, stop_event);
+}
+
+static int check_image_device(int fmt, const void* d_img, const void* d_blocks, unsigned width, unsigned height,
+                              size_t pitch)
+{
+    if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
+    if (width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
+    if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
+    if (width == 0 || height == 0) return GOOFY_OK;
+    if (!d_img || !d_blocks) return fail(GOOFY_ERR_INVALID, "null buffer");
+    if (pitch < (size_t)width * 4 || pitch % 16 != 0)
+        return fail(GOOFY_ERR_INVALID, "pitch must be >= 4*width and a multiple of 16");
+    if ((uintptr_t)d_img % 16 != 0 || (uintptr_t)d_blocks % 8 != 0)
+        return fail(GOOFY_ERR_INVALID, "image must be 16-byte and blocks 8-byte aligned");
+    const uint64_t bw = width / 4, nb = bw * (height / 4);
+    if (!magic_ok(bw, nb) || nb > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
+    return GOOFY_OK;
+}
+
+int goofy_hip_decode_device(int format, void* d_rgba, const void* d_blocks, unsigned int width, unsigned int height,
+                            size_t pitch, void* stream)
+{
+    clear_error();
+    int rc = check_image_device(format, d_rgba, d_blocks, width, height, pitch);
+    if (rc != GOOFY_OK || width == 0 || height == 0) return rc;
+    hipError_t e = launch_decode(format, d_blocks, d_rgba, width, height, pitch, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "decode kernel launch");
+    return GOOFY_OK;
+}
+
+static double psnr_of(double mse, double peak)
+{
+    if (mse < 0.0000001) return DBL_MAX;
+    return 10.0 * std::log10((peak * peak) / mse);
+}
+
+int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_source, unsigned int width,
+                             unsigned int height, size_t pitch, void* stream, goofy_quality* out)
+{
+    clear_error();
+    if (!out) return fail(GOOFY_ERR_INVALID, "null output");
+    int rc = check_image_device(format, d_source, d_blocks, width, height, pitch);
+    if (rc != GOOFY_OK) return rc;
+    if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, 64);
+    if (e != hipSuccess) return fail_hip(e, "hipMalloc(quality sums)");
+    unsigned long long* sums = static_cast<unsigned long long*>(d);
+    double* sum_y = reinterpret_cast<double*>(sums + 4);
+    struct Free {
+        void* p;
+        ~Free() { (void)hipFree(p); }
+    } guard{d};
+    e = launch_sqerr(format, d_blocks, d_source, pitch, width, height, sums, sum_y, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "quality kernel launch");
+    unsigned long long h[5];
+    e = hipMemcpyAsync(h, d, 40, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "quality readback");
+    goofy_quality q{};
+    q.pixels = (uint64_t)width * height;
+    for (int i = 0; i < 4; ++i) q.sse_byte[i] = h[i];
+    std::memcpy(&q.sse_luma, &h[4], sizeof(double));
+    const double n = (double)q.pixels;
+    // labels as in the reference harness: "r" = byte 2, "b" = byte 0
+    q.mse_r = (double)q.sse_byte[2] / n;
+    q.mse_g = (double)q.sse_byte[1] / n;
+    q.mse_b = (double)q.sse_byte[0] / n;
+    q.mse_a = (double)q.sse_byte[3] / n;
+    q.mse_rgb = (double)(q.sse_byte[0] + q.sse_byte[1] + q.sse_byte[2]) / n;
+    q.mse_y = q.sse_luma / n;
+    q.mse_max = std::max(std::max(q.mse_r, q.mse_g), std::max(q.mse_b, q.mse_a));
+    q.psnr_r = psnr_of(q.mse_r, 255.0);
+    q.psnr_g = psnr_of(q.mse_g, 255.0);
+    q.psnr_b = psnr_of(q.mse_b, 255.0);
+    q.psnr_a = psnr_of(q.mse_a, 255.0);
+    q.psnr_min = std::min(std::min(q.psnr_r, q.psnr_g), std::min(q.psnr_b, q.psnr_a));
+    q.psnr_rgb = psnr_of(q.mse_rgb, 768.0);
+    q.psnr_y = psnr_of(q.mse_y, 255.0);
+    *out = q;
+    return GOOFY_OK;
 }
 
 int goofy_synth_fill_host(int kind, uint64_t seed, uint32_t texture_id, uint32_t noise, unsigned char* dst,

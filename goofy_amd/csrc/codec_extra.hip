@@ -1,0 +1,244 @@
+// codec_extra.hip -- the other side of the encoder: BC1 / ETC1 block decoders
+// and a fused decode-and-compare kernel that gives the round-trip quality
+// (per-channel squared error, luma error, PSNR) of encoded blocks against
+// their source image.
+//
+// Semantics follow the decoders the reference's quality harness uses:
+// BC1 from Src/decoder.cpp:819-871 (5/6-bit expansion by bit replication,
+// thirds by truncating integer division, 3-colour mode when c0 <= c1 with
+// transparent black), ETC1 from Src/decoder.cpp:388-678 (individual and
+// differential modes, eight intensity tables, MSB/LSB selector planes in
+// column-major pixel order, flip bit).  ETC1 blocks whose differential sums
+// leave 0..31 are ETC2 T/H/planar blocks, which the ETC1 encoder never
+// produces; they are decoded here with 5-bit wrap-around and are outside the
+// parity claim.  The quality figures mirror getMsePsnr (Src/main.cpp:403-469).
+//
+// Layout as in kernels.hip: one lane per 4x4 block, a wave covers 64
+// consecutive blocks; 8-byte block loads and 16-byte pixel-row accesses are
+// coalesced per wave.
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace goofy_impl {
+
+namespace {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint2 load_block(const uint2* p)
+{
+    const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+    return make_uint2(v.x, v.y);
+}
+
+__device__ __forceinline__ uint32_t pack_rgba(uint32_t r, uint32_t g, uint32_t b, uint32_t a)
+{
+    return r | (g << 8) | (b << 16) | (a << 24);
+}
+
+// 16 decoded pixels (row-major) of a BC1 block.
+__device__ __forceinline__ void decode_bc1(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
+{
+    const uint32_t c0 = w0 & 0xFFFFu, c1 = w0 >> 16;
+    const uint32_t r0 = (c0 >> 11) & 31u, g0 = (c0 >> 5) & 63u, b0 = c0 & 31u;
+    const uint32_t r1 = (c1 >> 11) & 31u, g1 = (c1 >> 5) & 63u, b1 = c1 & 31u;
+    const uint32_t R0 = (r0 << 3) | (r0 >> 2), G0 = (g0 << 2) | (g0 >> 4), B0 = (b0 << 3) | (b0 >> 2);
+    const uint32_t R1 = (r1 << 3) | (r1 >> 2), G1 = (g1 << 2) | (g1 >> 4), B1 = (b1 << 3) | (b1 >> 2);
+    uint32_t pal[4];
+    pal[0] = pack_rgba(R0, G0, B0, 255u);
+    pal[1] = pack_rgba(R1, G1, B1, 255u);
+    if (c0 > c1) {
+        pal[2] = pack_rgba((2u * R0 + R1) / 3u, (2u * G0 + G1) / 3u, (2u * B0 + B1) / 3u, 255u);
+        pal[3] = pack_rgba((R0 + 2u * R1) / 3u, (G0 + 2u * G1) / 3u, (B0 + 2u * B1) / 3u, 255u);
+    } else {
+        pal[2] = pack_rgba((R0 + R1) / 2u, (G0 + G1) / 2u, (B0 + B1) / 2u, 255u);
+        pal[3] = 0u;
+    }
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const uint32_t k = (w1 >> (2 * p)) & 3u;
+        const uint32_t lo = (k & 1u) ? pal[1] : pal[0];
+        const uint32_t hi = (k & 1u) ? pal[3] : pal[2];
+        px[p] = (k & 2u) ? hi : lo;
+    }
+}
+
+__device__ __forceinline__ uint32_t clamp_add(uint32_t c, int32_t d)
+{
+    const int32_t v = (int32_t)c + d;
+    return (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+}
+
+// 16 decoded pixels (row-major) of an ETC1 block; w0/w1 are the block's
+// first/second 4 bytes read as little-endian words.
+__device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
+{
+    const uint32_t hi = __builtin_bswap32(w0), lo = __builtin_bswap32(w1);  // big-endian words
+    const bool diff = (hi >> 1) & 1u, flip = hi & 1u;
+    uint32_t base[2][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t field = (hi >> (24 - 8 * c)) & 0xFFu;
+        if (diff) {
+            const uint32_t a5 = field >> 3;
+            const int32_t d = (int32_t)(field & 7u) - ((field & 4u) ? 8 : 0);
+            const uint32_t b5 = (uint32_t)((int32_t)a5 + d) & 31u;
+            base[0][c] = (a5 << 3) | (a5 >> 2);
+            base[1][c] = (b5 << 3) | (b5 >> 2);
+        } else {
+            const uint32_t a4 = field >> 4, b4 = field & 15u;
+            base[0][c] = (a4 << 4) | a4;
+            base[1][c] = (b4 << 4) | b4;
+        }
+    }
+    const uint32_t cw[2] = {(hi >> 5) & 7u, (hi >> 2) & 7u};
+    uint32_t pal[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        // ETC1 intensity modifiers {small, large} per table index, one byte each:
+        // {2,8},{5,17},{9,29},{13,42},{18,60},{24,80},{33,106},{47,183}
+        const uint32_t a = (uint32_t)((0x2F2118120D090502ull >> (8 * cw[s])) & 0xFFu);
+        const uint32_t b = (uint32_t)((0xB76A503C2A1D1108ull >> (8 * cw[s])) & 0xFFu);
+        const int32_t mods[4] = {(int32_t)a, (int32_t)b, -(int32_t)a, -(int32_t)b};  // selector 0..3
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            pal[s][k] = pack_rgba(clamp_add(base[s][0], mods[k]), clamp_add(base[s][1], mods[k]),
+                                  clamp_add(base[s][2], mods[k]), 255u);
+    }
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            const int k = 4 * x + y;
+            const uint32_t sel = (((lo >> (16 + k)) & 1u) << 1) | ((lo >> k) & 1u);
+            const bool second = flip ? (y >= 2) : (x >= 2);
+            const uint32_t* P = second ? pal[1] : pal[0];
+            const uint32_t l = (sel & 1u) ? P[1] : P[0];
+            const uint32_t h = (sel & 1u) ? P[3] : P[2];
+            px[4 * y + x] = (sel & 2u) ? h : l;
+        }
+}
+
+template <int FMT>
+__device__ __forceinline__ void decode_block(uint2 b, uint32_t (&px)[16])
+{
+    if constexpr (FMT == 0) decode_bc1(b.x, b.y, px);
+    else decode_etc1(b.x, b.y, px);
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void k_decode(const uint2* __restrict__ blocks, uint8_t* __restrict__ dst,
+                                                uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nblocks) return;
+    const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
+    uint32_t px[16];
+    decode_block<FMT>(load_block(blocks + i), px);
+    uint8_t* row = dst + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const u32x4 v = {px[4 * r], px[4 * r + 1], px[4 * r + 2], px[4 * r + 3]};
+        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(row + (uint64_t)r * pitch));
+    }
+}
+
+// Round-trip error: decode each block in registers and compare it with the
+// source pixels.  sums[0..3] = squared error of bytes 0..3 of the pixels
+// (uint64), sums_y = sum of squared luma differences with the harness's
+// weights (0.21, 0.72, 0.07 on bytes 2, 1, 0 -- getMsePsnr reads a pixel as
+// a little-endian word and takes bits 16..23 as "red").
+template <int FMT>
+__global__ __launch_bounds__(256) void k_sqerr(const uint2* __restrict__ blocks, const uint8_t* __restrict__ src,
+                                               uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
+                                               unsigned long long* __restrict__ sums, double* __restrict__ sums_y)
+{
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t e[4] = {0, 0, 0, 0};
+    double ey = 0.0;
+    if (i < nblocks) {
+        const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
+        const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
+        uint32_t px[16];
+        decode_block<FMT>(load_block(blocks + i), px);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
+            const uint32_t s[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t a = s[c], d = px[4 * r + c];
+#pragma unroll
+                for (int ch = 0; ch < 4; ++ch) {
+                    const int32_t diff = (int32_t)((a >> (8 * ch)) & 0xFFu) - (int32_t)((d >> (8 * ch)) & 0xFFu);
+                    e[ch] += (uint32_t)(diff * diff);
+                }
+                const double y1 = 0.21 * (double)((a >> 16) & 0xFFu) + 0.72 * (double)((a >> 8) & 0xFFu) +
+                                  0.07 * (double)(a & 0xFFu);
+                const double y2 = 0.21 * (double)((d >> 16) & 0xFFu) + 0.72 * (double)((d >> 8) & 0xFFu) +
+                                  0.07 * (double)(d & 0xFFu);
+                const double dy = y2 - y1;
+                ey += dy * dy;
+            }
+        }
+    }
+    // wave reduction, one atomic per wave and sum
+    unsigned long long w[4] = {e[0], e[1], e[2], e[3]};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) w[ch] += __shfl_xor(w[ch], off, 64);
+        ey += __shfl_xor(ey, off, 64);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) atomicAdd(&sums[ch], w[ch]);
+        atomicAdd(sums_y, ey);
+    }
+}
+
+__global__ void k_zero_sums(unsigned long long* sums, double* sums_y)
+{
+    if (threadIdx.x < 4) sums[threadIdx.x] = 0ull;
+    if (threadIdx.x == 4) *sums_y = 0.0;
+}
+
+}  // namespace
+
+hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width, uint32_t height, size_t pitch,
+                         hipStream_t stream)
+{
+    const uint32_t bw = width / 4, nblocks = bw * (height / 4);
+    if (nblocks == 0) return hipSuccess;
+    const dim3 grid((nblocks + 255u) / 256u), block(256);
+    const uint64_t magic = div_magic(bw);
+    if (fmt == 0)
+        hipLaunchKernelGGL(k_decode<0>, grid, block, 0, stream, (const uint2*)blocks, (uint8_t*)dst,
+                           (uint64_t)pitch, bw, nblocks, magic);
+    else
+        hipLaunchKernelGGL(k_decode<1>, grid, block, 0, stream, (const uint2*)blocks, (uint8_t*)dst,
+                           (uint64_t)pitch, bw, nblocks, magic);
+    return hipGetLastError();
+}
+
+hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
+                        unsigned long long* d_sums, double* d_sum_y, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_sums, dim3(1), dim3(64), 0, stream, d_sums, d_sum_y);
+    const uint32_t bw = width / 4, nblocks = bw * (height / 4);
+    if (nblocks == 0) return hipGetLastError();
+    const dim3 grid((nblocks + 255u) / 256u), block(256);
+    const uint64_t magic = div_magic(bw);
+    if (fmt == 0)
+        hipLaunchKernelGGL(k_sqerr<0>, grid, block, 0, stream, (const uint2*)blocks, (const uint8_t*)src,
+                           (uint64_t)pitch, bw, nblocks, magic, d_sums, d_sum_y);
+    else
+        hipLaunchKernelGGL(k_sqerr<1>, grid, block, 0, stream, (const uint2*)blocks, (const uint8_t*)src,
+                           (uint64_t)pitch, bw, nblocks, magic, d_sums, d_sum_y);
+    return hipGetLastError();
+}
+
+}  // namespace goofy_impl

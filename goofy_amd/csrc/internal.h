@@ -49,8 +49,8 @@ hipError_t launch_synth(const void* params, void* dst, uint32_t width, uint32_t 
                         hipStream_t stream);
 hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width, uint32_t height, size_t pitch,
                          hipStream_t stream);
-hipError_t launch_sqerr(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width, uint32_t height,
-                        unsigned long long* d_sums, hipStream_t stream);
+hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
+                        unsigned long long* d_sums, double* d_sum_y, hipStream_t stream);
 
 // Error reporting: every C-ABI function stores a message for its thread.
 int fail(int code, const std::string& msg);

@@ -251,6 +251,8 @@ void goofy_server::run()
 {
     (void)hipSetDevice(device);
     const int max_batch = cfg.batching ? cfg.max_batch_textures : 1 << 30;
+    constexpr double kSpinUs = 2000.0;
+    double idle_since = -1;
     std::vector<Req> group;
     while (!stop.load(std::memory_order_acquire)) {
         bool progress = false;
@@ -296,17 +298,24 @@ void goofy_server::run()
             launch_group(*free_slot, group);
             progress = true;
         }
-        if (!progress) {
-            bool any_busy = false;
-            for (Slot& s : slots) any_busy |= s.busy;
-            if (any_busy) {
-                std::this_thread::yield();  // poll the in-flight events again
-            } else {
-                std::unique_lock<std::mutex> lock(mu);
-                cv_work.wait_for(lock, std::chrono::microseconds(200), [&] {
-                    return !incoming.empty() || stop.load(std::memory_order_acquire);
-                });
-            }
+        if (progress) {
+            idle_since = -1;
+            continue;
+        }
+        bool any_busy = false;
+        for (Slot& s : slots) any_busy |= s.busy;
+        const double t = now_us();
+        if (any_busy || idle_since < 0 || t - idle_since < kSpinUs) {
+            // poll events / the incoming queue again; spinning a little before
+            // sleeping keeps wake-up latency off the request path
+            if (!any_busy && idle_since < 0) idle_since = t;
+            std::this_thread::yield();
+        } else {
+            std::unique_lock<std::mutex> lock(mu);
+            cv_work.wait_for(lock, std::chrono::microseconds(500), [&] {
+                return !incoming.empty() || stop.load(std::memory_order_acquire);
+            });
+            idle_since = -1;
         }
     }
     for (Slot& s : slots)
@@ -367,10 +376,8 @@ int goofy_server_create(const goofy_server_config* ucfg, goofy_server** out)
     return GOOFY_OK;
 }
 
-int goofy_server_submit(goofy_server* s, const goofy_request* r, uint64_t* ticket)
+static int validate_request(const goofy_server* s, const goofy_request* r, Req* q)
 {
-    clear_error();
-    if (!s || !r) return fail(GOOFY_ERR_INVALID, "null argument");
     if (r->format != GOOFY_FORMAT_DXT1 && r->format != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
     if (r->width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
     if (r->height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
@@ -380,27 +387,49 @@ int goofy_server_submit(goofy_server* s, const goofy_request* r, uint64_t* ticke
         return fail(GOOFY_ERR_INVALID, "pitch must be >= 4*width and a multiple of 16");
     const uint64_t bw = r->width / 4, nb = bw * (r->height / 4);
     if (!magic_ok(bw, nb) || nb > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "texture too large");
-    Req q;
-    q.r = *r;
-    q.in_bytes = r->pitch * r->height;
-    q.out_bytes = nb * 8;
-    if (q.in_bytes + align_up(q.out_bytes) > s->cfg.hbm_budget_bytes || align_up(q.out_bytes) > s->arena.size())
+    q->r = *r;
+    q->in_bytes = r->pitch * r->height;
+    q->out_bytes = nb * 8;
+    if (q->in_bytes + align_up(q->out_bytes) > s->cfg.hbm_budget_bytes || align_up(q->out_bytes) > s->arena.size())
         return fail(GOOFY_ERR_INVALID, "request larger than the server's budget or arena");
+    return GOOFY_OK;
+}
+
+int goofy_server_submit_many(goofy_server* s, const goofy_request* reqs, int n, uint64_t* tickets, int* accepted)
+{
+    clear_error();
+    if (!s || (n > 0 && !reqs) || !accepted) return fail(GOOFY_ERR_INVALID, "null argument");
+    *accepted = 0;
+    int rc = GOOFY_OK;
     {
         std::lock_guard<std::mutex> lock(s->mu);
-        if (s->st.submitted - s->st.admitted >= (uint64_t)s->cfg.max_pending) {
-            ++s->st.rejected;
-            return fail(GOOFY_ERR_BUSY, "pending queue full");
+        const double t = s->now_us();
+        for (int i = 0; i < n; ++i) {
+            Req q;
+            if ((rc = validate_request(s, &reqs[i], &q)) != GOOFY_OK) break;
+            if (s->st.submitted - s->st.admitted >= (uint64_t)s->cfg.max_pending) {
+                ++s->st.rejected;
+                rc = fail(GOOFY_ERR_BUSY, "pending queue full");
+                break;
+            }
+            q.ticket = s->next_ticket++;
+            q.t_submit = t;
+            if (tickets) tickets[i] = q.ticket;
+            s->incoming.push_back(q);
+            ++s->st.submitted;
+            ++s->unfinished;
+            ++*accepted;
         }
-        q.ticket = s->next_ticket++;
-        q.t_submit = s->now_us();
-        s->incoming.push_back(q);
-        ++s->st.submitted;
-        ++s->unfinished;
     }
-    s->cv_work.notify_one();
-    if (ticket) *ticket = q.ticket;
-    return GOOFY_OK;
+    if (*accepted) s->cv_work.notify_one();
+    return rc;
+}
+
+int goofy_server_submit(goofy_server* s, const goofy_request* r, uint64_t* ticket)
+{
+    if (!r) return fail(GOOFY_ERR_INVALID, "null argument");
+    int accepted = 0;
+    return goofy_server_submit_many(s, r, 1, ticket, &accepted);
 }
 
 int goofy_server_poll(goofy_server* s, goofy_completion* out, int max, int* count)
@@ -417,11 +446,9 @@ int goofy_server_poll(goofy_server* s, goofy_completion* out, int max, int* coun
     return GOOFY_OK;
 }
 
-int goofy_server_release(goofy_server* s, uint64_t ticket)
+// caller holds s->mu
+static int release_locked(goofy_server* s, uint64_t ticket)
 {
-    clear_error();
-    if (!s) return fail(GOOFY_ERR_INVALID, "null server");
-    std::lock_guard<std::mutex> lock(s->mu);
     auto it = s->outstanding.find(ticket);
     if (it == s->outstanding.end()) return fail(GOOFY_ERR_INVALID, "unknown or already released ticket");
     if (!it->second.done) return fail(GOOFY_ERR_BUSY, "request not completed yet");
@@ -430,6 +457,27 @@ int goofy_server_release(goofy_server* s, uint64_t ticket)
     s->outstanding.erase(it);
     ++s->st.released;
     return GOOFY_OK;
+}
+
+int goofy_server_release(goofy_server* s, uint64_t ticket)
+{
+    clear_error();
+    if (!s) return fail(GOOFY_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    return release_locked(s, ticket);
+}
+
+int goofy_server_release_many(goofy_server* s, const uint64_t* tickets, int n)
+{
+    clear_error();
+    if (!s || (n > 0 && !tickets)) return fail(GOOFY_ERR_INVALID, "null argument");
+    int rc = GOOFY_OK;
+    std::lock_guard<std::mutex> lock(s->mu);
+    for (int i = 0; i < n; ++i) {
+        const int r = release_locked(s, tickets[i]);
+        if (r != GOOFY_OK && rc == GOOFY_OK) rc = r;
+    }
+    return rc;
 }
 
 int goofy_server_wait_idle(goofy_server* s, int timeout_ms)
@@ -515,10 +563,12 @@ int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_resu
     // completion consumer: latency bookkeeping, sampled verification, release
     std::thread consumer([&] {
         std::vector<goofy_completion> buf(1024);
+        std::vector<uint64_t> done;
         uint64_t seen = 0;
         while (true) {
             int n = 0;
             goofy_server_poll(srv, buf.data(), (int)buf.size(), &n);
+            done.clear();
             for (int i = 0; i < n; ++i) {
                 const goofy_completion& c = buf[i];
                 lat.push_back(c.t_done_us - c.t_submit_us);
@@ -538,8 +588,11 @@ int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_resu
                 }
                 ++seen;
                 ++completed;
-                goofy_server_release(srv, c.ticket);
-                outstanding.fetch_sub(1);
+                done.push_back(c.ticket);
+            }
+            if (n) {
+                goofy_server_release_many(srv, done.data(), (int)done.size());
+                outstanding.fetch_sub((uint64_t)n);
             }
             if (n == 0) {
                 if (gen_done.load() && outstanding.load() == 0) break;
@@ -553,31 +606,46 @@ int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_resu
     const double t_end = t_start + cfg->duration_s * 1e6;
     double next_arrival = t_start;
     double offered_px = 0;
+    std::vector<goofy_request> chunk;
+    std::vector<uint64_t> tickets;
     while (true) {
-        double now = goofy_server_now_us(srv);
+        const double now = goofy_server_now_us(srv);
         if (now >= t_end) break;
+        chunk.clear();
         if (cfg->rate_per_s > 0) {
-            if (now < next_arrival) {
+            // every arrival that is due by now, in one call
+            while (next_arrival <= now && chunk.size() < 4096) {
+                const int k = (int)(next_u64() % (uint64_t)cfg->pool_count);
+                const goofy_texture_desc& t = cfg->pool[k];
+                chunk.push_back(goofy_request{t.src, t.width, t.height, t.pitch, t.format, (uint64_t)k});
+                next_arrival += -std::log(next_unit()) / cfg->rate_per_s * 1e6;
+            }
+            if (chunk.empty()) {
                 if (next_arrival - now > 50.0) std::this_thread::sleep_for(std::chrono::microseconds(20));
                 continue;
             }
-            next_arrival += -std::log(next_unit()) / cfg->rate_per_s * 1e6;
-        } else if (outstanding.load() >= (uint64_t)cfg->closed_loop_depth) {
-            std::this_thread::yield();
-            continue;
-        }
-        const int k = (int)(next_u64() % (uint64_t)cfg->pool_count);
-        const goofy_texture_desc& t = cfg->pool[k];
-        goofy_request r{t.src, t.width, t.height, t.pitch, t.format, (uint64_t)k};
-        uint64_t ticket = 0;
-        outstanding.fetch_add(1);
-        const int s = goofy_server_submit(srv, &r, &ticket);
-        if (s == GOOFY_OK) {
-            ++submitted;
-            offered_px += (double)t.width * t.height;
         } else {
-            outstanding.fetch_sub(1);
-            ++rejected;
+            const uint64_t out = outstanding.load();
+            if (out >= (uint64_t)cfg->closed_loop_depth) {
+                std::this_thread::yield();
+                continue;
+            }
+            const uint64_t room = std::min<uint64_t>((uint64_t)cfg->closed_loop_depth - out, 256);
+            for (uint64_t j = 0; j < room; ++j) {
+                const int k = (int)(next_u64() % (uint64_t)cfg->pool_count);
+                const goofy_texture_desc& t = cfg->pool[k];
+                chunk.push_back(goofy_request{t.src, t.width, t.height, t.pitch, t.format, (uint64_t)k});
+            }
+        }
+        tickets.resize(chunk.size());
+        outstanding.fetch_add(chunk.size());
+        int accepted = 0;
+        const int s = goofy_server_submit_many(srv, chunk.data(), (int)chunk.size(), tickets.data(), &accepted);
+        submitted += (uint64_t)accepted;
+        for (int j = 0; j < accepted; ++j) offered_px += (double)chunk[j].width * chunk[j].height;
+        if (accepted < (int)chunk.size()) {
+            outstanding.fetch_sub(chunk.size() - (size_t)accepted);
+            rejected += chunk.size() - (size_t)accepted;
             if (s != GOOFY_ERR_BUSY) {
                 rc = s;
                 break;

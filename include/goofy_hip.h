@@ -89,6 +89,41 @@ GOOFY_HIP_API int goofy_hip_compress_device_timed(int format, void* d_result, co
                                                   unsigned int width, unsigned int height, size_t pitch, void* stream,
                                                   const goofy_launch_config* cfg, void* start_event, void* stop_event);
 
+/* ---- decode and round-trip quality ------------------------------------------
+ * Decoders with the semantics of the reference's quality harness
+ * (Src/decoder.cpp: BC1 :819-871, ETC1 :388-678).  `d_rgba` receives RGBA8
+ * rows of `pitch` bytes (multiple of 16, >= 4*width, 16-byte aligned);
+ * `d_blocks` holds (width/4)*(height/4) tightly packed 8-byte blocks.
+ * ETC1 blocks whose differential sums leave 0..31 (ETC2 T/H/planar) are not
+ * ETC1 and are decoded with 5-bit wrap-around. */
+GOOFY_HIP_API int goofy_hip_decode_device(int format, void* d_rgba, const void* d_blocks, unsigned int width,
+                                          unsigned int height, size_t pitch, void* stream);
+
+/* Round-trip error of encoded blocks against their source image, with the
+ * definitions of the reference harness's getMsePsnr (Src/main.cpp:403-469):
+ * a pixel is read as a little-endian word, "r" = bits 16..23, "g" = 8..15,
+ * "b" = 0..7, "a" = 24..31; luma = 0.21 r + 0.72 g + 0.07 b; PSNR =
+ * 10 log10(peak^2 / mse) with peak 255, and 768 for the summed-RGB figure;
+ * an MSE below 1e-7 gives DBL_MAX.  Synchronous on `stream`; allocates a
+ * small device buffer per call. */
+typedef struct goofy_quality {
+    uint64_t pixels;
+    uint64_t sse_byte[4]; /* squared error of bytes 0..3 of each pixel, summed */
+    double sse_luma;
+    double mse_r, mse_g, mse_b, mse_a, mse_max, mse_rgb, mse_y;
+    double psnr_r, psnr_g, psnr_b, psnr_a, psnr_min, psnr_rgb, psnr_y;
+} goofy_quality;
+
+GOOFY_HIP_API int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_source, unsigned int width,
+                                           unsigned int height, size_t pitch, void* stream, goofy_quality* out);
+
+/* Roofline calibration: the plain encode kernel's memory traffic (same
+ * addresses, tiles and streaming hints; 64 bytes read and 8 written per
+ * block) with no encode arithmetic.  Its time is the floor for the encode
+ * kernels on the device at hand.  Events as in goofy_hip_compress_device_timed. */
+GOOFY_HIP_API int goofy_hip_traffic_probe(void* d_out, const void* d_input, unsigned int width, unsigned int height,
+                                          size_t pitch, void* stream, void* start_event, void* stop_event);
+
 /* ---- batched encode: many textures, one launch -----------------------------
  * A plan flattens the block work-lists of n device-resident textures (each
  * its own size, pitch and format) into one wave-tile list and encodes all of
@@ -175,10 +210,17 @@ GOOFY_HIP_API int goofy_server_create(const goofy_server_config* cfg, goofy_serv
 /* Thread-safe.  GOOFY_ERR_BUSY when the pending queue is full, GOOFY_ERR_INVALID
  * for a malformed request or one larger than the budget or the arena. */
 GOOFY_HIP_API int goofy_server_submit(goofy_server* server, const goofy_request* request, uint64_t* ticket);
+/* Submits requests in order under one lock; stops at the first one refused
+ * (its status is returned) and sets *accepted to the number taken.  tickets[i]
+ * receives the ticket of request i when accepted (tickets may be NULL). */
+GOOFY_HIP_API int goofy_server_submit_many(goofy_server* server, const goofy_request* requests, int count,
+                                           uint64_t* tickets, int* accepted);
 /* Up to `max` completions, oldest first; *count receives how many. */
 GOOFY_HIP_API int goofy_server_poll(goofy_server* server, goofy_completion* out, int max, int* count);
 /* Frees the request's output range and its budget share. */
 GOOFY_HIP_API int goofy_server_release(goofy_server* server, uint64_t ticket);
+/* Releases many tickets under one lock; returns the first error, if any. */
+GOOFY_HIP_API int goofy_server_release_many(goofy_server* server, const uint64_t* tickets, int count);
 /* Waits until every submitted request has completed (timeout_ms < 0: forever). */
 GOOFY_HIP_API int goofy_server_wait_idle(goofy_server* server, int timeout_ms);
 GOOFY_HIP_API int goofy_server_stats_get(goofy_server* server, goofy_server_stats* stats);

@@ -130,7 +130,16 @@ class Reference:
         lib.goofy_ref_compress_mt.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.c_uint, ctypes.c_uint,
                                               ctypes.c_uint, ctypes.c_int]
         lib.goofy_ref_encode_blocks.argtypes = [ctypes.c_int, _u8p, ctypes.c_size_t, _u8p]
+        lib.goofy_ref_decode_block.argtypes = [ctypes.c_int, _u8p, _u8p]
         self.lib = lib
+
+    def decode_blocks(self, fmt: int, blocks: np.ndarray) -> np.ndarray:
+        """Reference decoders (Src/decoder.cpp) on (n, 8) blocks -> (n, 64) RGBA8."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 8)
+        out = np.zeros((blocks.shape[0], 64), dtype=np.uint8)
+        for i in range(blocks.shape[0]):
+            self.lib.goofy_ref_decode_block(fmt, _ptr(blocks[i]), _ptr(out[i]))
+        return out
 
     def encode_blocks(self, fmt: int, blocks: np.ndarray) -> np.ndarray:
         blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)

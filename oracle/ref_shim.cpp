@@ -6,13 +6,16 @@
 // repository).  The result, oracle/_ref/libgoofy_ref.so, is used to pin the
 // C restatement (oracle/goofy_oracle.c), to write the golden vectors under
 // tests/golden/, and as the CPU baseline of the benchmark (the reference SSE2
-// path, one horizontal stripe of block rows per host thread).
+// path, one horizontal stripe of block rows per host thread).  The
+// reference's block decoders (Src/decoder.cpp, compiled from where it lies)
+// are wrapped too, to pin the decoder oracle and the GPU decoders.
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
 
+#include "decoder.h"
 #include "goofy_tc.h"
 
 namespace {
@@ -94,6 +97,15 @@ int goofy_ref_encode_blocks(int fmt, const unsigned char* blocks, size_t n, unsi
     memcpy(out, res.data(), n * 8);
     free(img);
     return rc;
+}
+
+// Decode one 8-byte block into 16 RGBA8 pixels (row-major, 64 bytes) with
+// the reference harness's decoders (DecoderBC::decodeBlockDXT1 / ETC1,
+// Src/decoder.cpp:929-971).
+void goofy_ref_decode_block(int fmt, const unsigned char* block, unsigned char* out)
+{
+    if (fmt == 0) DecoderBC::decodeBlockDXT1(block, out, 16);
+    else DecoderBC::decodeBlockETC1(block, out, 16);
 }
 
 }  // extern "C"

@@ -1,0 +1,97 @@
+"""GPU BC1/ETC1 decoders and the round-trip quality kernel, through the C-ABI.
+
+Decoders: bit-exact against the decoder oracle, which is itself pinned to the
+reference harness's decoders (tests/test_oracle.py).  Quality: integer
+squared errors exact; the luma error is a double-precision sum in a different
+order than the reference's serial loop, so it is compared with relative
+tolerance 1e-9 (and the PSNR figures to 1e-7 dB).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from goofy_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def random_etc1(rng, n):
+    out = []
+    while len(out) < n:
+        b = rng.integers(0, 256, 8, dtype=np.uint8)
+        if b[3] & 2 and not all(0 <= (int(v) >> 3) + ((int(v) & 7) - 8 * ((int(v) & 7) >= 4)) <= 31 for v in b[:3]):
+            continue
+        out.append(b)
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("fmt", [W.DXT1, W.ETC1])
+def test_decode_matches_oracle(gpu, oracle, fmt):
+    from goofy_amd import hip
+    rng = np.random.default_rng(7 + fmt)
+    bw, bh = 100, 37
+    n = bw * bh
+    blocks = rng.integers(0, 256, (n, 8), dtype=np.uint8) if fmt == W.DXT1 else random_etc1(rng, n)
+    w, h = 4 * bw, 4 * bh
+    pitch = w * 4 + 48
+    d_blocks = hip.DeviceBuffer.from_array(blocks.reshape(-1))
+    d_img = hip.DeviceBuffer(pitch * h, fill=0x3C)
+    print(f"decode fmt={fmt} {w}x{h} pitch={pitch}", flush=True)
+    gpu.decode_device(fmt, d_img, d_blocks, w, h, pitch)
+    hip.synchronize()
+    got = d_img.download().reshape(h, pitch)
+    want = oracle.decode_image(fmt, blocks.reshape(-1), w, h).reshape(h, w * 4)
+    assert np.array_equal(got[:, : w * 4], want)
+    assert (got[:, w * 4:] == 0x3C).all()  # padding untouched
+
+
+def reference_quality(src, dec):
+    """getMsePsnr (Src/main.cpp:403-469) restated in numpy float64."""
+    a = src.reshape(-1, 4).astype(np.float64)
+    b = dec.reshape(-1, 4).astype(np.float64)
+    n = a.shape[0]
+    se = ((a - b) ** 2).sum(0)
+    y1 = 0.21 * a[:, 2] + 0.72 * a[:, 1] + 0.07 * a[:, 0]
+    y2 = 0.21 * b[:, 2] + 0.72 * b[:, 1] + 0.07 * b[:, 0]
+    mse_y = ((y2 - y1) ** 2).sum() / n
+    psnr = lambda m, p=255.0: float("inf") if m < 1e-7 else 10 * math.log10(p * p / m)  # noqa: E731
+    return {"sse_byte": [int(x) for x in se], "mse_y": mse_y, "psnr_y": psnr(mse_y),
+            "psnr_rgb": psnr(se[:3].sum() / n, 768.0), "psnr_r": psnr(se[2] / n)}
+
+
+@pytest.mark.parametrize("fmt", [W.DXT1, W.ETC1])
+def test_quality_matches_harness_definition(gpu, oracle, fmt):
+    from goofy_amd import hip
+    im = W.IMAGE_512
+    img = gpu.synth_host(im.kind, im.seed, im.texture_id, im.noise, im.width, im.height, im.pitch)
+    src = hip.DeviceBuffer.from_array(img)
+    blocks = hip.DeviceBuffer(im.blocks * 8)
+    gpu.compress_device(fmt, blocks, src, im.width, im.height, im.pitch)
+    q = gpu.quality_device(fmt, blocks, src, im.width, im.height, im.pitch)
+    enc = blocks.download()
+    dec = oracle.decode_image(fmt, enc, im.width, im.height)
+    ref = reference_quality(img, dec)
+    print(f"quality fmt={fmt}: psnr_rgb={q['psnr_rgb']:.3f} psnr_y={q['psnr_y']:.3f}", flush=True)
+    assert q["pixels"] == im.pixels
+    assert q["sse_byte"] == ref["sse_byte"]
+    assert q["mse_y"] == pytest.approx(ref["mse_y"], rel=1e-9)
+    assert q["psnr_y"] == pytest.approx(ref["psnr_y"], abs=1e-7)
+    assert q["psnr_rgb"] == pytest.approx(ref["psnr_rgb"], abs=1e-7)
+    assert q["psnr_r"] == pytest.approx(ref["psnr_r"], abs=1e-7)
+    assert q["mse_a"] == 0.0 and q["psnr_a"] > 1e300  # alpha is 255 in and out
+
+
+def test_quality_of_black_image_closed_form(gpu):
+    from goofy_amd import hip
+    # black 16x16: c0 = 0x0020 (the forced green LSB), c1 = 0, every index 2,
+    # so every pixel decodes to (0, (2*4+0)/3, 0) = (0, 2, 0): squared error 4
+    # on byte 1 only; alpha is exact
+    img = np.tile(np.array([0, 0, 0, 255], np.uint8), 16 * 16)
+    src = hip.DeviceBuffer.from_array(img)
+    blocks = hip.DeviceBuffer(16 * 8)
+    gpu.compress_device(W.DXT1, blocks, src, 16, 16, 64)
+    q = gpu.quality_device(W.DXT1, blocks, src, 16, 16, 64)
+    assert q["sse_byte"] == [0, 4 * 256, 0, 0]
+    assert q["psnr_g"] == pytest.approx(10 * math.log10(255.0 ** 2 / 4.0), abs=1e-12)
+    assert q["psnr_min"] == q["psnr_g"] and q["psnr_a"] > 1e300 and q["psnr_r"] > 1e300

@@ -261,3 +261,33 @@ def test_driver_return_codes(oracle, reference):
     assert oracle.compress(W.ETC1, img, 0, 64, 0)[0] == 0
     for args in ((60, 64, 240), (64, 62, 256), (60, 62, 240), (0, 64, 0)):
         assert reference.compress(W.DXT1, img, *args)[0] == oracle.compress(W.DXT1, img, *args)[0]
+
+
+# ---- decoder oracle vs the reference's decoders (Src/decoder.cpp) --------------------------
+
+def valid_etc1_blocks(rng, n):
+    """Random ETC1 blocks: individual mode, or differential mode whose sums stay in 0..31."""
+    out = []
+    while len(out) < n:
+        b = rng.integers(0, 256, 8, dtype=np.uint8)
+        if b[3] & 2:
+            ok = all(0 <= (int(v) >> 3) + ((int(v) & 7) - 8 * ((int(v) & 7) >= 4)) <= 31 for v in b[:3])
+            if not ok:
+                continue
+        out.append(b)
+    return np.stack(out)
+
+
+def test_decoder_oracle_matches_reference_bc1(oracle, reference):
+    rng = np.random.default_rng(41)
+    blocks = rng.integers(0, 256, (20000, 8), dtype=np.uint8)  # both 4- and 3-colour modes
+    assert np.array_equal(oracle.decode_blocks(W.DXT1, blocks), reference.decode_blocks(W.DXT1, blocks))
+
+
+def test_decoder_oracle_matches_reference_etc1(oracle, reference):
+    rng = np.random.default_rng(43)
+    blocks = valid_etc1_blocks(rng, 20000)
+    assert (blocks[:, 3] & 2).any() and not (blocks[:, 3] & 2).all()
+    assert np.array_equal(oracle.decode_blocks(W.ETC1, blocks), reference.decode_blocks(W.ETC1, blocks))
+    enc = oracle.encode_blocks(W.ETC1, rng.integers(0, 256, (5000, 64), dtype=np.uint8))
+    assert np.array_equal(oracle.decode_blocks(W.ETC1, enc), reference.decode_blocks(W.ETC1, enc))
