@@ -174,6 +174,27 @@ def bench_batch(g, hip, W, reps: int = 7):
     return res
 
 
+# ---- drop-in with host buffers (PCIe-inclusive) -------------------------------------------------
+
+def bench_dropin(g, W, reps: int = 5):
+    """goofy::compressDXT1/ETC1 on host memory: staging, kernel and copy-back, as a caller
+    of the reference API sees it.  Best of `reps` per format and size."""
+    res = {}
+    for im in (W.IMAGE_512, W.IMAGE_8192):
+        img = g.synth_host(im.kind, im.seed, im.texture_id, im.noise, im.width, im.height, im.pitch)
+        out = np.empty(im.blocks * 8, np.uint8)
+        for f, key, fn in ((W.DXT1, "dxt1", g.compressDXT1), (W.ETC1, "etc1", g.compressETC1)):
+            assert fn(out, img, im.width, im.height, im.pitch) == 0
+            best = 1e9
+            for _ in range(reps if im is W.IMAGE_8192 else 4 * reps):
+                t0 = time.perf_counter()
+                fn(out, img, im.width, im.height, im.pitch)
+                best = min(best, time.perf_counter() - t0)
+            res[f"{im.name}_{key}"] = {"ms": round(best * 1e3, 3), "mps": round(im.pixels / 1e6 / best, 1),
+                                        "pcie_gbps": round((im.pitch * im.height + im.blocks * 8) / best / 1e9, 2)}
+    return res
+
+
 # ---- CPU baseline ------------------------------------------------------------------------------
 
 def cpu_baseline(g, W, budget_s: float = 3.0):
@@ -295,6 +316,7 @@ def main() -> int:
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-dropin", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--profile", type=int, default=0, help="rocprof mode: N launches per kernel, no JSON")
     ap.add_argument("--sweep", action="store_true", help="launch-shape sweep table, no JSON")
@@ -358,6 +380,9 @@ def main() -> int:
     if not args.skip_serve:
         from goofy_amd import serving
         result["served"] = serving.bench_serving(g, hip, W, log=log, duration_s=args.serve_seconds)
+    if not args.skip_dropin:
+        log("drop-in on host buffers")
+        result["dropin_host"] = bench_dropin(g, W)
     if not args.skip_cpu:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(g, W)

@@ -256,6 +256,18 @@ typedef struct goofy_serve_sim_result {
 
 GOOFY_HIP_API int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_result* result);
 
+/* ---- containers ----------------------------------------------------------------
+ * DDS (DXT1) and KTX1 (ETC1 or DXT1) files around (width/4)*(height/4)*8 block
+ * bytes, with the headers the reference's harness writes (saveDds / saveKtx,
+ * Src/main.cpp:154-220).  Host memory only; no GPU needed. */
+enum { GOOFY_CONTAINER_DDS = 0, GOOFY_CONTAINER_KTX = 1 };
+GOOFY_HIP_API size_t goofy_container_bytes(int kind, unsigned int width, unsigned int height);
+GOOFY_HIP_API int goofy_container_write(int kind, int format, unsigned int width, unsigned int height,
+                                        const void* blocks, size_t blocks_bytes, void* out, size_t out_capacity,
+                                        size_t* written);
+GOOFY_HIP_API int goofy_container_write_file(const char* path, int kind, int format, unsigned int width,
+                                             unsigned int height, const void* blocks, size_t blocks_bytes);
+
 /* ---- synthetic inputs ------------------------------------------------------
  * Seeded, integer-only texture generator (host and device give identical
  * bytes).  kind 0: three 2-D sinusoids per channel + uniform noise in

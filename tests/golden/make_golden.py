@@ -7,6 +7,12 @@ and records what it outputs:
   blocks.npz   -- special and random 4x4 blocks (64 B each) with the
                   reference's DXT1 and ETC1S outputs (8 B each);
   img512.npz   -- the full DXT1 / ETC1S outputs of the 512x512 waves image;
+  real_images.json -- for three of the reference's own test images
+                  (test-data/parrot_red.png, kodim23.png, patterns.png, copied
+                  under tests/golden/images/ as data): SHA-256 of the RGBA8
+                  input as the harness loads it and of both encodings, and the
+                  harness's PSNR figures of the round trip through the
+                  reference's decoders;
   digests.json -- SHA-256 of inputs and outputs of the 512^2, 8192^2 and the
                   pitch-padded 16384^2 stress images, and of a 16-texture
                   batch with widths that are multiples of 4 but not of 16
@@ -157,7 +163,36 @@ def main() -> int:
         digests["batch"].append({**t, "input_sha256": sha(img), "output_sha256": sha(out)})
     (OUT / "digests.json").write_text(json.dumps(digests, indent=1) + "\n")
     print("digests.json written")
+
+    from goofy_amd.io import load_png_rgba8
+    real = {}
+    for name in ("parrot_red.png", "kodim23.png", "patterns.png"):
+        img, w, h = load_png_rgba8(OUT / "images" / name)
+        entry = {"width": w, "height": h, "input_sha256": sha(img)}
+        for fmt, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+            out = encode_image_ref(ref, fmt, img, w, h, w * 4)
+            entry[f"{key}_sha256"] = sha(out)
+            dec = ref.decode_blocks(fmt, out.reshape(-1, 8))  # (n, 64): blocks row-major
+            dec = dec.reshape(h // 4, w // 4, 4, 16).transpose(0, 2, 1, 3).reshape(-1)
+            entry[f"{key}_psnr"] = harness_psnr(img, dec)
+        real[name] = entry
+        print(name, w, h, {k: v for k, v in entry.items() if k.endswith("psnr")})
+    (OUT / "real_images.json").write_text(json.dumps(real, indent=1) + "\n")
     return 0
+
+
+def harness_psnr(src: np.ndarray, dec: np.ndarray) -> dict:
+    """getMsePsnr (Src/main.cpp:403-469): pixel read as a little-endian word,
+    "r" = bits 16..23, "b" = bits 0..7; RGB PSNR with peak 768."""
+    a = src.reshape(-1, 4).astype(np.float64)
+    b = dec.reshape(-1, 4).astype(np.float64)
+    n = a.shape[0]
+    se = ((a - b) ** 2).sum(0)
+    y1 = 0.21 * a[:, 2] + 0.72 * a[:, 1] + 0.07 * a[:, 0]
+    y2 = 0.21 * b[:, 2] + 0.72 * b[:, 1] + 0.07 * b[:, 0]
+    mse_y = float(((y2 - y1) ** 2).sum() / n)
+    psnr = lambda m, p=255.0: None if m < 1e-7 else 10 * np.log10(p * p / m)  # noqa: E731
+    return {"sse_byte": [int(x) for x in se], "psnr_rgb": psnr(se[:3].sum() / n, 768.0), "psnr_y": psnr(mse_y)}
 
 
 if __name__ == "__main__":

@@ -252,3 +252,30 @@ def test_batch_full_workload(gpu, oracle):
         want = oracle.compress_blocks(it["format"], img, it["width"], it["height"], it["pitch"])
         assert np.array_equal(first[int(i)], want), it["meta"]
     b.close()
+
+
+# ---- real images (the reference's own test data) -------------------------------------------
+
+def _real():
+    import json
+    from conftest import GOLDEN
+    return GOLDEN, json.loads((GOLDEN / "real_images.json").read_text())
+
+
+@pytest.mark.parametrize("name", ["parrot_red.png", "kodim23.png", "patterns.png"])
+def test_real_images_gpu(gpu, name):
+    from goofy_amd import io as gio
+    golden, real = _real()
+    g = real[name]
+    img, w, h = gio.load_png_rgba8(golden / "images" / name)
+    src = upload(img)
+    for fmt, key, fn in ((W.DXT1, "dxt1", gpu.compressDXT1), (W.ETC1, "etc1", gpu.compressETC1)):
+        say(f"real image {name} {key}")
+        assert sha(encode_dev(gpu, fmt, src, w, h, w * 4)) == g[f"{key}_sha256"]
+        out = np.zeros((w // 4) * (h // 4) * 8, np.uint8)
+        assert fn(out, img, w, h, w * 4) == 0 and sha(out) == g[f"{key}_sha256"]
+        blocks = upload(out)
+        q = gpu.quality_device(fmt, blocks, src, w, h, w * 4)
+        assert q["sse_byte"] == g[f"{key}_psnr"]["sse_byte"]
+        assert q["psnr_rgb"] == pytest.approx(g[f"{key}_psnr"]["psnr_rgb"], abs=1e-9)
+        assert q["psnr_y"] == pytest.approx(g[f"{key}_psnr"]["psnr_y"], abs=1e-7)
