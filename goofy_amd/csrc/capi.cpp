@@ -4,10 +4,14 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -15,6 +19,7 @@
 
 #include "goofy_hip.h"
 #include "goofy_tc.h"
+#include "copy_pool.h"
 #include "internal.h"
 #include "synth.h"
 
@@ -82,20 +87,27 @@ bool cfg_valid(const LaunchCfg& c)
     return bpl && wg && c.waves_per_cu >= 0 && c.waves_per_cu <= 256 && (c.nontemporal == 0 || c.nontemporal == 1);
 }
 
-// ---- staging contexts for the host-memory drop-in ------------------------------
+// ---- host-memory drop-in: pinned staging, chunked pipeline ---------------------
 //
-// A context is a stream plus grow-only device buffers.  Contexts live in a
-// process-wide pool: a call takes one, uses it, and gives it back, so
-// concurrent callers never share a stream or a buffer, and nothing is freed
+// goofy::compressDXT1/ETC1 take pageable host buffers.  The image is moved in
+// chunks of block rows (about 8 MiB of pixels each) through two pinned
+// staging slots: while the GPU copies chunk k in, encodes it and copies its
+// blocks out, host threads already copy the rows of chunk k+1 into the other
+// slot.  Row copies of large chunks are split over a small persistent thread
+// pool.  Contexts (stream, events, device and pinned buffers) come from a
+// process-wide pool: concurrent callers never share one, and nothing is freed
 // at thread exit (the HIP runtime may already be gone then).
 
 struct StagingCtx {
     int device = -1;
     hipStream_t stream = nullptr;
-    void* d_in = nullptr;
-    size_t in_cap = 0;
-    void* d_out = nullptr;
-    size_t out_cap = 0;
+    hipEvent_t h2d_done[2] = {nullptr, nullptr};   // pinned input slot may be refilled
+    hipEvent_t out_ready[2] = {nullptr, nullptr};  // pinned output slot holds a finished chunk
+    void* d_in[2] = {nullptr, nullptr};
+    void* d_out[2] = {nullptr, nullptr};
+    void* h_in[2] = {nullptr, nullptr};
+    void* h_out[2] = {nullptr, nullptr};
+    size_t in_cap = 0, out_cap = 0;
 };
 
 class StagingPool {
@@ -109,9 +121,9 @@ public:
                 free_.erase(free_.begin() + (long)i);
                 return c;
             }
-        all_.push_back(new StagingCtx());
-        all_.back()->device = device;
-        return all_.back();
+        StagingCtx* c = new StagingCtx();
+        c->device = device;
+        return c;
     }
     void release(StagingCtx* c)
     {
@@ -121,7 +133,7 @@ public:
 
 private:
     std::mutex mu_;
-    std::vector<StagingCtx*> all_, free_;
+    std::vector<StagingCtx*> free_;
 };
 
 static StagingPool& staging_pool()
@@ -130,22 +142,61 @@ static StagingPool& staging_pool()
     return *pool;
 }
 
-static int ensure(void** buf, size_t* cap, size_t need, const char* what)
+static int ensure_ctx(StagingCtx* c, size_t in_bytes, size_t out_bytes)
 {
-    if (*cap >= need) return GOOFY_OK;
-    if (*buf) {
-        (void)hipFree(*buf);
-        *buf = nullptr;
-        *cap = 0;
+    hipError_t e;
+    if (!c->stream) {
+        if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+            return fail_hip(e, "hipStreamCreate");
+        for (int i = 0; i < 2; ++i) {
+            if ((e = hipEventCreateWithFlags(&c->h2d_done[i], hipEventDisableTiming)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&c->out_ready[i], hipEventDisableTiming)) != hipSuccess)
+                return fail_hip(e, "hipEventCreate");
+        }
     }
-    const size_t sz = std::max<size_t>(need, 1u << 20);
-    hipError_t e = hipMalloc(buf, sz);
-    if (e != hipSuccess) {
-        *buf = nullptr;
-        return fail_hip(e, what);
+    if (c->in_cap < in_bytes || c->out_cap < out_bytes) {
+        (void)hipStreamSynchronize(c->stream);
+        const size_t ni = std::max(in_bytes, c->in_cap), no = std::max(out_bytes, c->out_cap);
+        for (int i = 0; i < 2; ++i) {
+            if (c->d_in[i]) (void)hipFree(c->d_in[i]);
+            if (c->d_out[i]) (void)hipFree(c->d_out[i]);
+            if (c->h_in[i]) (void)hipHostFree(c->h_in[i]);
+            if (c->h_out[i]) (void)hipHostFree(c->h_out[i]);
+            c->d_in[i] = c->d_out[i] = c->h_in[i] = c->h_out[i] = nullptr;
+        }
+        c->in_cap = c->out_cap = 0;
+        for (int i = 0; i < 2; ++i) {
+            if ((e = hipMalloc(&c->d_in[i], ni)) != hipSuccess || (e = hipMalloc(&c->d_out[i], no)) != hipSuccess)
+                return fail_hip(e, "hipMalloc(staging)");
+            if ((e = hipHostMalloc(&c->h_in[i], ni, hipHostMallocDefault)) != hipSuccess ||
+                (e = hipHostMalloc(&c->h_out[i], no, hipHostMallocDefault)) != hipSuccess)
+                return fail_hip(e, "hipHostMalloc(staging)");
+        }
+        c->in_cap = ni;
+        c->out_cap = no;
     }
-    *cap = sz;
     return GOOFY_OK;
+}
+
+// Copy `rows` rows of `row_bytes` from src (stride) to dst (packed), split
+// over the copy pool when large.
+static void copy_rows(uint8_t* dst, const uint8_t* src, size_t stride, size_t row_bytes, unsigned rows)
+{
+    const size_t total = row_bytes * rows;
+    CopyPool& pool = CopyPool::get();
+    const int parts = total >= (2u << 20) ? std::min<int>(pool.workers() + 1, (int)std::max<size_t>(1, total >> 20)) : 1;
+    if (stride == row_bytes) {
+        const size_t step = (total + parts - 1) / parts;
+        pool.run(parts, [&](int i) {
+            const size_t a = (size_t)i * step, b = std::min(total, a + step);
+            if (a < b) std::memcpy(dst + a, src + a, b - a);
+        });
+        return;
+    }
+    pool.run(parts, [&](int i) {
+        const unsigned r0 = (unsigned)((uint64_t)rows * i / parts), r1 = (unsigned)((uint64_t)rows * (i + 1) / parts);
+        for (unsigned r = r0; r < r1; ++r) std::memcpy(dst + r * row_bytes, src + (size_t)r * stride, row_bytes);
+    });
 }
 
 static int compress_host(int fmt, unsigned char* result, const unsigned char* input, unsigned width,
@@ -171,24 +222,48 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
         StagingCtx* c;
         ~Release() { staging_pool().release(c); }
     } rel{ctx};
-    hipError_t e;
-    if (!ctx->stream) {
-        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) return fail_hip(e, "hipStreamCreate");
-    }
-    const size_t pitch = (size_t)width * 4;
-    const size_t in_bytes = pitch * height, out_bytes = nblocks * 8;
-    if ((rc = ensure(&ctx->d_in, &ctx->in_cap, in_bytes, "hipMalloc(input staging)")) != GOOFY_OK) return rc;
-    if ((rc = ensure(&ctx->d_out, &ctx->out_cap, out_bytes, "hipMalloc(output staging)")) != GOOFY_OK) return rc;
 
-    e = hipMemcpy2DAsync(ctx->d_in, pitch, input, stride, pitch, height, hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return fail_hip(e, "hipMemcpy2DAsync(H2D)");
-    e = launch_plain(fmt, default_plain_cfg(), ctx->d_in, pitch, ctx->d_out, bw, (uint32_t)nblocks, ctx->stream);
-    if (e != hipSuccess) return fail_hip(e, "encode kernel launch");
-    e = hipMemcpyAsync(result, ctx->d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream);
-    if (e != hipSuccess) return fail_hip(e, "hipMemcpyAsync(D2H)");
-    e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    const size_t row_bytes = (size_t)width * 4;
+    const unsigned chunk_brows = (unsigned)std::max<size_t>(1, (size_t(8) << 20) / (row_bytes * 4));  // ~8 MiB
+    const unsigned cb = std::min(chunk_brows, bh);
+    if ((rc = ensure_ctx(ctx, row_bytes * 4 * cb, (size_t)bw * cb * 8)) != GOOFY_OK) return rc;
+    const unsigned nchunks = (bh + cb - 1) / cb;
+    const LaunchCfg cfg = default_plain_cfg();
+    hipError_t e = hipSuccess;
+
+    auto finish_chunk = [&](unsigned k) -> hipError_t {
+        const int s = (int)(k & 1u);
+        const unsigned rows = std::min(cb, bh - k * cb);
+        hipError_t r = hipEventSynchronize(ctx->out_ready[s]);
+        if (r != hipSuccess) return r;
+        std::memcpy(result + (size_t)k * cb * bw * 8, ctx->h_out[s], (size_t)rows * bw * 8);
+        return hipSuccess;
+    };
+
+    for (unsigned k = 0; k < nchunks && e == hipSuccess; ++k) {
+        const int s = (int)(k & 1u);
+        const unsigned rows = std::min(cb, bh - k * cb);  // block rows in this chunk
+        if (k >= 2 && (e = hipEventSynchronize(ctx->h2d_done[s])) != hipSuccess) break;
+        copy_rows(static_cast<uint8_t*>(ctx->h_in[s]), input + (size_t)k * cb * 4 * stride, stride, row_bytes,
+                  rows * 4);
+        if ((e = hipMemcpyAsync(ctx->d_in[s], ctx->h_in[s], row_bytes * 4 * rows, hipMemcpyHostToDevice,
+                                ctx->stream)) != hipSuccess)
+            break;
+        if ((e = hipEventRecord(ctx->h2d_done[s], ctx->stream)) != hipSuccess) break;
+        if ((e = launch_plain(fmt, cfg, ctx->d_in[s], row_bytes, ctx->d_out[s], bw, bw * rows, ctx->stream)) !=
+            hipSuccess)
+            break;
+        if ((e = hipMemcpyAsync(ctx->h_out[s], ctx->d_out[s], (size_t)bw * rows * 8, hipMemcpyDeviceToHost,
+                                ctx->stream)) != hipSuccess)
+            break;
+        if ((e = hipEventRecord(ctx->out_ready[s], ctx->stream)) != hipSuccess) break;
+        if (k >= 1) e = finish_chunk(k - 1);
+    }
+    if (e == hipSuccess) e = finish_chunk(nchunks - 1);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return fail_hip(e, "drop-in staging pipeline");
+    }
     return GOOFY_OK;
 }
 
