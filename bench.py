@@ -2,13 +2,15 @@
 
 Headline (`value`): plain encode of one 8192x8192 RGBA8 image (seeded waves +
 noise 12) already resident in HBM; one step = the DXT1 encode plus the ETC1S
-encode of that image (one kernel launch each), so MP/s counts the pixels of
-both formats per second of wall time over the timed steps.  Four input
-buffers of 256 MiB are rotated so that no launch reads an image the 256 MB
-last-level cache still holds.
+encode of that image (one kernel launch each; the two independent launches
+go to two streams), so MP/s counts the pixels of both formats per second of
+wall time over the timed steps.  Four input buffers of 256 MiB are rotated so
+that no launch reads an image the 256 MB last-level cache still holds.
 
-Also reported: per-format kernel times (HIP events on the launch stream) and
-the HBM roofline of the slower kernel; the batched workload (4096 textures,
+Also reported, from a second timed region of the same steps on one stream:
+per-format kernel times (HIP events recorded by each kernel's own dispatch)
+and the HBM roofline of the slower kernel, beside a traffic probe (same
+addresses, no encode arithmetic) that gives the attainable floor; the batched workload (4096 textures,
 one persistent graph launch vs 4096 plain launches); the serving workload
 (Poisson arrivals, p50/p99 latency); and the reference encoder on the host
 CPU (compiled from its own source, one stripe of block rows per thread).
@@ -66,34 +68,47 @@ def bench_plain(g, hip, W, steps: int, warmup: int, rotate: int, cfg: dict | Non
     parity["input"] = sha(srcs[0].download()) == digests["input_sha256"]
 
     j = 0
+    stream_b = hip.Stream()
 
     def step(events=None):
+        """One DXT1 and one ETC1S encode.  Timed for the roofline: both on one
+        stream, each with start/stop events recorded by its own dispatch.
+        Timed for throughput: the two independent launches on two streams."""
         nonlocal j
         for f in (W.DXT1, W.ETC1):
             k = j % rotate
             if events is not None:
-                # start/stop recorded by the kernel's own dispatch packet
                 events[f].append((hip.Event(), hip.Event()))
                 g.compress_device_timed(f, outs[f][k], srcs[k], im.width, im.height, im.pitch,
                                         events[f][-1][0], events[f][-1][1], stream, cfg)
             else:
-                g.compress_device(f, outs[f][k], srcs[k], im.width, im.height, im.pitch, stream, cfg)
+                g.compress_device(f, outs[f][k], srcs[k], im.width, im.height, im.pitch,
+                                  stream if f == W.DXT1 else stream_b, cfg)
             j += 1
 
     log(f"plain: warmup {warmup} steps")
     for _ in range(warmup):
         step()
     stream.synchronize()
-    # timed region: wall clock around exactly `steps` steps, events per launch
-    events = {W.DXT1: [], W.ETC1: []}
+    stream_b.synchronize()
+    # timed region 1 (value): wall clock around exactly `steps` steps
     log(f"plain: timing {steps} steps")
     t0 = time.perf_counter()
     for _ in range(steps):
+        step()
+    stream.synchronize()
+    stream_b.synchronize()
+    elapsed = time.perf_counter() - t0
+    # timed region 2 (roofline): the same steps on one stream, per-kernel events
+    events = {W.DXT1: [], W.ETC1: []}
+    t1 = time.perf_counter()
+    for _ in range(steps):
         step(events)
     stream.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed_single = time.perf_counter() - t1
     kern_ms = {f: float(np.mean([a.elapsed_ms(b) for a, b in events[f]])) for f in events}
     kern_med = {f: float(np.median([a.elapsed_ms(b) for a, b in events[f]])) for f in events}
+    kern_ms["single_stream_ms_per_step"] = elapsed_single / steps * 1e3
     # the same traffic without the encode arithmetic: the attainable floor
     probe = []
     for i in range(2 * steps):
@@ -105,7 +120,39 @@ def bench_plain(g, hip, W, steps: int, warmup: int, rotate: int, cfg: dict | Non
     for b in srcs + outs[W.DXT1] + outs[W.ETC1]:
         b.free()
     stream.destroy()
+    stream_b.destroy()
     return im, elapsed, kern_ms, kern_med, parity
+
+
+# ---- branch-coverage stress image ---------------------------------------------------------------
+
+def bench_stress(g, hip, W, launches: int = 20):
+    """16384x16384 stress texture (64x64 regions: constant, checker, saturated
+    primaries, noise, normal map), pitch 65792 B: both formats, kernel time from
+    dispatch-recorded events, output digests checked against the reference's."""
+    im = W.STRESS_16384
+    digests = json.loads((ROOT / "tests" / "golden" / "digests.json").read_text())["images"][im.name]
+    stream = hip.Stream()
+    src = hip.DeviceBuffer(im.pitch * im.height, fill=0)
+    g.synth_device(im.kind, im.seed, im.texture_id, im.noise, src, im.width, im.height, im.pitch, stream)
+    out = hip.DeviceBuffer(im.blocks * 8)
+    algo = W.algorithmic_bytes(im.width, im.height)
+    res = {"workload": f"{im.name} RGBA8 (1 GiB of pixels), stress regions, both formats"}
+    for f, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+        g.compress_device(f, out, src, im.width, im.height, im.pitch, stream)
+        stream.synchronize()
+        exact = sha(out.download()) == digests[f"{key}_sha256"]
+        ev = [(hip.Event(), hip.Event()) for _ in range(launches)]
+        for a, b in ev:
+            g.compress_device_timed(f, out, src, im.width, im.height, im.pitch, a, b, stream)
+        stream.synchronize()
+        ms = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
+        res[key] = {"kernel_ms_mean": round(ms, 4), "mps": round(im.pixels / 1e6 / (ms / 1e3), 1),
+                    "gbps": round(algo / (ms / 1e3) / 1e9, 1), "bit_exact": exact}
+    src.free()
+    out.free()
+    stream.destroy()
+    return res
 
 
 # ---- batched --------------------------------------------------------------------------------
@@ -317,6 +364,7 @@ def main() -> int:
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-dropin", action="store_true")
+    ap.add_argument("--skip-stress", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--profile", type=int, default=0, help="rocprof mode: N launches per kernel, no JSON")
     ap.add_argument("--sweep", action="store_true", help="launch-shape sweep table, no JSON")
@@ -366,15 +414,20 @@ def main() -> int:
         formats[key] = {"kernel_ms_mean": round(kern_ms[f], 5), "kernel_ms_median": round(kern_med[f], 5),
                         "kernel_mps": round(im.pixels / 1e6 / (kern_ms[f] / 1e3), 1),
                         "kernel_gbps": round(algo / (kern_ms[f] / 1e3) / 1e9, 1)}
+    formats["single_stream_ms_per_step"] = round(kern_ms["single_stream_ms_per_step"], 5)
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "MP/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": f"plain encode, {im.name} RGBA8 stride {im.pitch} B, waves+noise12 seed "
-                               f"{im.seed:#x}; step = DXT1 + ETC1S launch, inputs HBM-resident, 4-buffer rotation",
+                               f"{im.seed:#x}; step = DXT1 + ETC1S launch (two streams), inputs HBM-resident, "
+                               "4-buffer rotation",
                    "global_batch": 1, "formats": ["DXT1", "ETC1S"], "parallelism": "single GPU"},
         "roofline": roofline, "formats": formats, "parity": parity,
     }
+    if not args.skip_stress:
+        log("stress 16384^2")
+        result["stress"] = bench_stress(g, hip, W)
     if not args.skip_batch:
         result["batched"] = bench_batch(g, hip, W)
     if not args.skip_serve:

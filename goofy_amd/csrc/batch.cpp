@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -78,7 +79,13 @@ int build_worklist(goofy_batch* b)
         b->descs.push_back(d);
     }
     b->ntiles = (uint32_t)tiles;
-    const size_t bytes = std::max<size_t>(1, b->descs.size()) * sizeof(TexDesc);
+    const size_t ntex = b->descs.size();
+    std::vector<uint8_t> host(worklist_bytes(ntex));
+    if (ntex) std::memcpy(host.data(), b->descs.data(), ntex * sizeof(TexDesc));
+    uint32_t* tb = worklist_tile_begin(reinterpret_cast<TexDesc*>(host.data()), ntex);
+    for (size_t i = 0; i < ntex; ++i) tb[i] = b->descs[i].tile_begin;
+    tb[ntex] = b->ntiles;
+    const size_t bytes = host.size();
     hipError_t e;
     if (b->d_descs_cap < bytes) {
         if (b->d_descs) (void)hipFree(b->d_descs);
@@ -88,10 +95,8 @@ int build_worklist(goofy_batch* b)
         if (e != hipSuccess) return fail_hip(e, "hipMalloc(descriptors)");
         b->d_descs_cap = bytes;
     }
-    if (!b->descs.empty()) {
-        e = hipMemcpy(b->d_descs, b->descs.data(), b->descs.size() * sizeof(TexDesc), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return fail_hip(e, "hipMemcpy(descriptors)");
-    }
+    e = hipMemcpy(b->d_descs, host.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy(work-list)");
     drop_graph(b);
     return GOOFY_OK;
 }

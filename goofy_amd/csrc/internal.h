@@ -21,8 +21,8 @@ struct TexDesc {
     uint64_t magic;    // ceil(2^40 / bw): block row = (i * magic) >> 40
     uint32_t bw;       // blocks per row
     uint32_t nblocks;  // bw * bh
-    uint32_t tile_begin;
-    uint32_t fmt;      // GOOFY_FORMAT_DXT1 / GOOFY_FORMAT_ETC1
+    uint32_t tile_begin;  // also in the work-list's prefix-sum array
+    uint32_t fmt;         // GOOFY_FORMAT_DXT1 / GOOFY_FORMAT_ETC1
     uint32_t pad[4];
 };
 static_assert(sizeof(TexDesc) == 64, "TexDesc layout");
@@ -34,6 +34,15 @@ struct LaunchCfg {
     int waves_per_cu;     // persistent grid size (0 = one wave per tile, no persistence)
     int nontemporal;      // 1 = streaming (non-temporal) loads and stores
 };
+
+// A batch work-list in device memory: ntex descriptors followed by the
+// ntex+1 prefix sums of their tile counts (uint32, last = total tiles).
+inline size_t worklist_bytes(size_t ntex) { return ntex * sizeof(TexDesc) + (ntex + 1) * sizeof(uint32_t); }
+inline uint32_t* worklist_tile_begin(TexDesc* descs, size_t ntex) { return reinterpret_cast<uint32_t*>(descs + ntex); }
+inline const uint32_t* worklist_tile_begin(const TexDesc* descs, size_t ntex)
+{
+    return reinterpret_cast<const uint32_t*>(descs + ntex);
+}
 
 inline uint64_t div_magic(uint32_t bw) { return ((1ull << 40) + bw - 1) / bw; }
 

@@ -11,6 +11,8 @@
 // This replaces the reference driver loop (GoofyTC/goofy_tc.h:1514-1524,
 // 4 blocks per call) and goofySimdEncode (:1069-1494); the arithmetic of a
 // block is in block_codec.h.
+#include <algorithm>
+
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -85,31 +87,43 @@ __global__ __launch_bounds__(512) void k_encode_plain(const uint8_t* __restrict_
     encode_tile<FMT, BPL, NT>(src, pitch, dst, bw, magic, nblocks, wave * 64u * BPL, threadIdx.x & 63u);
 }
 
-// Many textures: waves stride over the flattened tile list (persistent when
-// the grid is smaller than the list).  The owning texture of a tile is found
-// by a wave-uniform binary search over the descriptors' tile_begin (scalar
-// loads); formats are uniform per tile, so the branch never diverges.
+// Many textures.  tile_begin[0..ntex] is the prefix sum of the textures'
+// tile counts (tile_begin[ntex] = ntiles); texture d owns tiles
+// [tile_begin[d], tile_begin[d+1]).  Wave w encodes the contiguous tile range
+// [w*tpw, (w+1)*tpw): one wave-uniform binary search over the compact
+// prefix-sum array (scalar loads) finds the first tile's texture, after which
+// the wave steps forward across texture boundaries.  tpw = 1 is one wave per
+// tile; tpw > 1 is a persistent grid.  Formats are uniform per tile, so the
+// format branch never diverges.
 template <int BPL, bool NT>
-__global__ __launch_bounds__(512) void k_encode_batch(const TexDesc* __restrict__ descs, uint32_t ntex,
-                                                      uint32_t ntiles)
+__global__ __launch_bounds__(512) void k_encode_batch(const TexDesc* __restrict__ descs,
+                                                      const uint32_t* __restrict__ tile_begin, uint32_t ntex,
+                                                      uint32_t ntiles, uint32_t tpw)
 {
-    const uint32_t wpg = blockDim.x >> 6;
-    const uint32_t nwaves = gridDim.x * wpg;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * wpg + (threadIdx.x >> 6)); t < ntiles;
-         t += nwaves) {
-        uint32_t lo = 0, hi = ntex;
-        while (hi - lo > 1u) {
-            const uint32_t m = (lo + hi) >> 1;
-            if (descs[m].tile_begin <= t) lo = m;
-            else hi = m;
+    uint32_t t = wave * tpw;
+    const uint32_t t_end = min(t + tpw, ntiles);
+    if (t >= t_end) return;
+    uint32_t lo = 0, hi = ntex;
+    while (hi - lo > 1u) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (tile_begin[m] <= t) lo = m;
+        else hi = m;
+    }
+    uint32_t d = lo, begin = tile_begin[d], next = tile_begin[d + 1];
+    for (; t < t_end; ++t) {
+        while (t >= next) {  // every texture owns at least one tile, and next <= ntiles
+            ++d;
+            begin = next;
+            next = tile_begin[d + 1];
         }
-        const TexDesc& d = descs[lo];
-        const uint32_t first = (t - d.tile_begin) * 64u * BPL;
-        const uint8_t* src = reinterpret_cast<const uint8_t*>(d.src);
-        uint8_t* dst = reinterpret_cast<uint8_t*>(d.dst);
-        if (d.fmt == 0) encode_tile<0, BPL, NT>(src, d.pitch, dst, d.bw, d.magic, d.nblocks, first, lane);
-        else encode_tile<1, BPL, NT>(src, d.pitch, dst, d.bw, d.magic, d.nblocks, first, lane);
+        const TexDesc& D = descs[d];
+        const uint32_t first = (t - begin) * 64u * BPL;
+        const uint8_t* src = reinterpret_cast<const uint8_t*>(D.src);
+        uint8_t* dst = reinterpret_cast<uint8_t*>(D.dst);
+        if (D.fmt == 0) encode_tile<0, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane);
+        else encode_tile<1, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane);
     }
 }
 
@@ -174,11 +188,11 @@ hipError_t launch_plain(int fmt, const LaunchCfg& c, const void* src, size_t pit
 }
 
 template <int BPL>
-static hipError_t batch_nt(bool nt, dim3 grid, dim3 block, const TexDesc* d, uint32_t ntex, uint32_t ntiles,
-                           hipStream_t s)
+static hipError_t batch_nt(bool nt, dim3 grid, dim3 block, const TexDesc* d, const uint32_t* tb, uint32_t ntex,
+                           uint32_t ntiles, uint32_t tpw, hipStream_t s)
 {
-    if (nt) hipLaunchKernelGGL((k_encode_batch<BPL, true>), grid, block, 0, s, d, ntex, ntiles);
-    else hipLaunchKernelGGL((k_encode_batch<BPL, false>), grid, block, 0, s, d, ntex, ntiles);
+    if (nt) hipLaunchKernelGGL((k_encode_batch<BPL, true>), grid, block, 0, s, d, tb, ntex, ntiles, tpw);
+    else hipLaunchKernelGGL((k_encode_batch<BPL, false>), grid, block, 0, s, d, tb, ntex, ntiles, tpw);
     return hipGetLastError();
 }
 
@@ -186,17 +200,17 @@ hipError_t launch_batch(const LaunchCfg& c, const TexDesc* d_descs, uint32_t nte
                         hipStream_t stream)
 {
     if (ntiles == 0 || ntex == 0) return hipSuccess;
-    const uint32_t wg = (uint32_t)c.wg_waves;
+    const uint32_t* tb = worklist_tile_begin(d_descs, ntex);
     uint64_t waves = ntiles;
-    if (c.waves_per_cu > 0) {
-        const uint64_t persistent = (uint64_t)num_cus * (uint64_t)c.waves_per_cu;
-        if (persistent < waves) waves = persistent;
-    }
+    if (c.waves_per_cu > 0) waves = std::min<uint64_t>(waves, (uint64_t)num_cus * (uint64_t)c.waves_per_cu);
+    const uint32_t tpw = (uint32_t)((ntiles + waves - 1) / waves);
+    waves = (ntiles + tpw - 1) / tpw;
+    const uint32_t wg = (uint32_t)c.wg_waves;
     const dim3 grid((uint32_t)((waves + wg - 1) / wg)), block(64u * wg);
     switch (c.blocks_per_lane) {
-    case 1: return batch_nt<1>(c.nontemporal, grid, block, d_descs, ntex, ntiles, stream);
-    case 2: return batch_nt<2>(c.nontemporal, grid, block, d_descs, ntex, ntiles, stream);
-    default: return batch_nt<4>(c.nontemporal, grid, block, d_descs, ntex, ntiles, stream);
+    case 1: return batch_nt<1>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, stream);
+    case 2: return batch_nt<2>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, stream);
+    default: return batch_nt<4>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, stream);
     }
 }
 

@@ -198,8 +198,10 @@ bool goofy_server::launch_group(Slot& s, std::vector<Req>& group)
             d.tile_begin = tiles;
             tiles += (uint32_t)((d.nblocks + tile - 1) / tile);
             s.h_desc[i] = d;
+            worklist_tile_begin(s.h_desc, group.size())[i] = d.tile_begin;
         }
-        e = hipMemcpyAsync(s.d_desc, s.h_desc, group.size() * sizeof(TexDesc), hipMemcpyHostToDevice, s.stream);
+        worklist_tile_begin(s.h_desc, group.size())[group.size()] = tiles;
+        e = hipMemcpyAsync(s.d_desc, s.h_desc, worklist_bytes(group.size()), hipMemcpyHostToDevice, s.stream);
         if (e == hipSuccess) e = launch_batch(launch, s.d_desc, (uint32_t)group.size(), tiles, num_cus, s.stream);
     } else {
         for (Req& q : group) {
@@ -359,7 +361,7 @@ int goofy_server_create(const goofy_server_config* ucfg, goofy_server** out)
         return fail(GOOFY_ERR_OUT_OF_MEMORY, "hipMalloc(output arena) failed");
     }
     s->slots.resize((size_t)c.streams);
-    const size_t desc_bytes = (size_t)c.max_batch_textures * sizeof(TexDesc);
+    const size_t desc_bytes = worklist_bytes((size_t)c.max_batch_textures);
     for (Slot& sl : s->slots) {
         hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
