@@ -46,6 +46,17 @@ def build_library(verbose: bool = True) -> Path:
     return LIB
 
 
+def build_examples(verbose: bool = True) -> None:
+    """tests/native/libdropin_example.so: a C++ caller of the drop-in API."""
+    out = ROOT / "tests" / "native" / "libdropin_example.so"
+    cmd = ["g++", "-std=c++17", "-O1", "-fPIC", "-shared", f"-I{ROOT / 'include'}",
+           str(ROOT / "tests" / "native" / "dropin_example.cpp"), f"-L{PKG}", "-lgoofy_hip",
+           f"-Wl,-rpath,{PKG}", "-o", str(out)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
 def build_oracle(verbose: bool = True) -> None:
     cmd = ["make", "-C", str(ROOT / "oracle"), "-s", "all"]
     if verbose:
@@ -55,6 +66,7 @@ def build_oracle(verbose: bool = True) -> None:
 
 def main(argv: list[str]) -> int:
     build_library()
+    build_examples()
     if "--no-oracle" not in argv:
         build_oracle()
     return 0
