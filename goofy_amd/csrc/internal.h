@@ -35,6 +35,15 @@ struct LaunchCfg {
     int nontemporal;      // 1 = streaming (non-temporal) loads and stores
 };
 
+// A small work-list passed by value in the kernel arguments (no copy to the
+// device before the launch): up to kInlineTextures descriptors and their
+// ntex+1 tile prefix sums.
+constexpr int kInlineTextures = 16;
+struct InlineWorklist {
+    TexDesc d[kInlineTextures];
+    uint32_t tile_begin[kInlineTextures + 1];
+};
+
 // A batch work-list in device memory: ntex descriptors followed by the
 // ntex+1 prefix sums of their tile counts (uint32, last = total tiles).
 inline size_t worklist_bytes(size_t ntex) { return ntex * sizeof(TexDesc) + (ntex + 1) * sizeof(uint32_t); }
@@ -54,6 +63,8 @@ hipError_t launch_plain(int fmt, const LaunchCfg& cfg, const void* src, size_t p
                         uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_batch(const LaunchCfg& cfg, const TexDesc* d_descs, uint32_t ntex, uint32_t ntiles, int num_cus,
                         hipStream_t stream);
+hipError_t launch_batch_inline(const LaunchCfg& cfg, const InlineWorklist& wl, uint32_t ntex, uint32_t ntiles,
+                               int num_cus, hipStream_t stream);
 hipError_t launch_synth(const void* params, void* dst, uint32_t width, uint32_t height, size_t pitch,
                         hipStream_t stream);
 hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width, uint32_t height, size_t pitch,

@@ -95,10 +95,9 @@ __global__ __launch_bounds__(512) void k_encode_plain(const uint8_t* __restrict_
 // the wave steps forward across texture boundaries.  tpw = 1 is one wave per
 // tile; tpw > 1 is a persistent grid.  Formats are uniform per tile, so the
 // format branch never diverges.
-template <int BPL, bool NT>
-__global__ __launch_bounds__(512) void k_encode_batch(const TexDesc* __restrict__ descs,
-                                                      const uint32_t* __restrict__ tile_begin, uint32_t ntex,
-                                                      uint32_t ntiles, uint32_t tpw)
+template <int BPL, bool NT, class Descs, class Begins>
+__device__ __forceinline__ void batch_body(const Descs& descs, const Begins& tile_begin, uint32_t ntex,
+                                           uint32_t ntiles, uint32_t tpw)
 {
     const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63u;
@@ -125,6 +124,22 @@ __global__ __launch_bounds__(512) void k_encode_batch(const TexDesc* __restrict_
         if (D.fmt == 0) encode_tile<0, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane);
         else encode_tile<1, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane);
     }
+}
+
+template <int BPL, bool NT>
+__global__ __launch_bounds__(512) void k_encode_batch(const TexDesc* __restrict__ descs,
+                                                      const uint32_t* __restrict__ tile_begin, uint32_t ntex,
+                                                      uint32_t ntiles, uint32_t tpw)
+{
+    batch_body<BPL, NT>(descs, tile_begin, ntex, ntiles, tpw);
+}
+
+// The same with the work-list in the kernel arguments (small groups).
+template <int BPL, bool NT>
+__global__ __launch_bounds__(512) void k_encode_batch_inline(const InlineWorklist wl, uint32_t ntex, uint32_t ntiles,
+                                                             uint32_t tpw)
+{
+    batch_body<BPL, NT>(wl.d, wl.tile_begin, ntex, ntiles, tpw);
 }
 
 __global__ __launch_bounds__(256) void k_synth(goofy_synth::Params P, uint8_t* __restrict__ dst, uint32_t width,
@@ -211,6 +226,33 @@ hipError_t launch_batch(const LaunchCfg& c, const TexDesc* d_descs, uint32_t nte
     case 1: return batch_nt<1>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, stream);
     case 2: return batch_nt<2>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, stream);
     default: return batch_nt<4>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, stream);
+    }
+}
+
+template <int BPL>
+static hipError_t batch_inline_nt(bool nt, dim3 grid, dim3 block, const InlineWorklist& wl, uint32_t ntex,
+                                  uint32_t ntiles, uint32_t tpw, hipStream_t s)
+{
+    if (nt) hipLaunchKernelGGL((k_encode_batch_inline<BPL, true>), grid, block, 0, s, wl, ntex, ntiles, tpw);
+    else hipLaunchKernelGGL((k_encode_batch_inline<BPL, false>), grid, block, 0, s, wl, ntex, ntiles, tpw);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_inline(const LaunchCfg& c, const InlineWorklist& wl, uint32_t ntex, uint32_t ntiles,
+                               int num_cus, hipStream_t stream)
+{
+    if (ntiles == 0 || ntex == 0) return hipSuccess;
+    if (ntex > (uint32_t)kInlineTextures) return hipErrorInvalidValue;
+    uint64_t waves = ntiles;
+    if (c.waves_per_cu > 0) waves = std::min<uint64_t>(waves, (uint64_t)num_cus * (uint64_t)c.waves_per_cu);
+    const uint32_t tpw = (uint32_t)((ntiles + waves - 1) / waves);
+    waves = (ntiles + tpw - 1) / tpw;
+    const uint32_t wg = (uint32_t)c.wg_waves;
+    const dim3 grid((uint32_t)((waves + wg - 1) / wg)), block(64u * wg);
+    switch (c.blocks_per_lane) {
+    case 1: return batch_inline_nt<1>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, stream);
+    case 2: return batch_inline_nt<2>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, stream);
+    default: return batch_inline_nt<4>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, stream);
     }
 }
 

@@ -183,6 +183,13 @@ bool goofy_server::launch_group(Slot& s, std::vector<Req>& group)
     const double t = now_us();
     hipError_t e = hipSuccess;
     if (cfg.batching) {
+        // Groups of up to kInlineTextures travel in the kernel arguments (no
+        // descriptor copy ahead of the kernel); larger ones through the slot's
+        // pinned staging and one host-to-device copy.
+        const bool inline_list = group.size() <= (size_t)kInlineTextures;
+        InlineWorklist wl;
+        TexDesc* descs = inline_list ? wl.d : s.h_desc;
+        uint32_t* begins = inline_list ? wl.tile_begin : worklist_tile_begin(s.h_desc, group.size());
         const uint64_t tile = (uint64_t)tile_blocks(launch);
         uint32_t tiles = 0;
         for (size_t i = 0; i < group.size(); ++i) {
@@ -197,12 +204,16 @@ bool goofy_server::launch_group(Slot& s, std::vector<Req>& group)
             d.fmt = (uint32_t)r.format;
             d.tile_begin = tiles;
             tiles += (uint32_t)((d.nblocks + tile - 1) / tile);
-            s.h_desc[i] = d;
-            worklist_tile_begin(s.h_desc, group.size())[i] = d.tile_begin;
+            descs[i] = d;
+            begins[i] = d.tile_begin;
         }
-        worklist_tile_begin(s.h_desc, group.size())[group.size()] = tiles;
-        e = hipMemcpyAsync(s.d_desc, s.h_desc, worklist_bytes(group.size()), hipMemcpyHostToDevice, s.stream);
-        if (e == hipSuccess) e = launch_batch(launch, s.d_desc, (uint32_t)group.size(), tiles, num_cus, s.stream);
+        begins[group.size()] = tiles;
+        if (inline_list) {
+            e = launch_batch_inline(launch, wl, (uint32_t)group.size(), tiles, num_cus, s.stream);
+        } else {
+            e = hipMemcpyAsync(s.d_desc, s.h_desc, worklist_bytes(group.size()), hipMemcpyHostToDevice, s.stream);
+            if (e == hipSuccess) e = launch_batch(launch, s.d_desc, (uint32_t)group.size(), tiles, num_cus, s.stream);
+        }
     } else {
         for (Req& q : group) {
             const uint32_t bw = q.r.width / 4;
