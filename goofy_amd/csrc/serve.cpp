@@ -150,10 +150,10 @@ struct goofy_server {
     double now_us() const { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
     void run();
     bool launch_group(Slot& s, std::vector<Req>& group);
-    void finish(Slot& s);
+    void finish(Slot& s, int status);
 };
 
-void goofy_server::finish(Slot& s)
+void goofy_server::finish(Slot& s, int status)
 {
     const double t = now_us();
     std::lock_guard<std::mutex> lock(mu);
@@ -163,7 +163,7 @@ void goofy_server::finish(Slot& s)
         goofy_completion c{};
         c.ticket = q.ticket;
         c.user_tag = q.r.user_tag;
-        c.status = GOOFY_OK;
+        c.status = status;
         c.dst = arena.base() + q.arena_off;
         c.dst_bytes = q.out_bytes;
         c.t_submit_us = q.t_submit;
@@ -269,11 +269,15 @@ void goofy_server::run()
     std::vector<Req> group;
     while (!stop.load(std::memory_order_acquire)) {
         bool progress = false;
-        for (Slot& s : slots)
-            if (s.busy && hipEventQuery(s.done) == hipSuccess) {
-                finish(s);
-                progress = true;
-            }
+        for (Slot& s : slots) {
+            if (!s.busy) continue;
+            const hipError_t q = hipEventQuery(s.done);
+            if (q == hipErrorNotReady) continue;
+            // a failed query completes the group with an error instead of
+            // leaving its slot busy forever
+            finish(s, q == hipSuccess ? GOOFY_OK : GOOFY_ERR_HIP);
+            progress = true;
+        }
         {
             std::lock_guard<std::mutex> lock(mu);
             while (!incoming.empty()) {
@@ -332,10 +336,7 @@ void goofy_server::run()
         }
     }
     for (Slot& s : slots)
-        if (s.busy) {
-            (void)hipEventSynchronize(s.done);
-            finish(s);
-        }
+        if (s.busy) finish(s, hipEventSynchronize(s.done) == hipSuccess ? GOOFY_OK : GOOFY_ERR_HIP);
 }
 
 extern "C" {
