@@ -279,3 +279,30 @@ def test_real_images_gpu(gpu, name):
         assert q["sse_byte"] == g[f"{key}_psnr"]["sse_byte"]
         assert q["psnr_rgb"] == pytest.approx(g[f"{key}_psnr"]["psnr_rgb"], abs=1e-9)
         assert q["psnr_y"] == pytest.approx(g[f"{key}_psnr"]["psnr_y"], abs=1e-7)
+
+
+def test_device_image_beyond_4gib(gpu, oracle):
+    """32768x32768 RGBA8 (4 GiB of pixels, 67M blocks): byte offsets past 2^32.
+    Stripes of block rows at the start, the middle and the very end are compared
+    with the oracle (blocks are independent, so a stripe's blocks are exactly
+    the encoding of that stripe as an image)."""
+    from goofy_amd import hip
+    w = h = 32768
+    pitch = w * 4
+    say("device 32768x32768 stress: synth on device (4 GiB)")
+    src = dev_buffer(pitch * h)
+    gpu.synth_device(1, 0x601F0099, 0, 0, src, w, h, pitch)
+    out = dev_buffer((w // 4) * (h // 4) * 8)
+    for fmt in (W.DXT1, W.ETC1):
+        say(f"device 32768x32768 fmt={fmt}")
+        gpu.compress_device(fmt, out, src, w, h, pitch)
+        hip.synchronize()
+        row_blocks = (w // 4) * 8
+        for by in (0, 4097, 6000, h // 4 - 2):
+            rows = 2
+            img = src.download(pitch * 4 * rows, offset=pitch * 4 * by)
+            want = oracle.compress_blocks(fmt, img, w, 4 * rows, pitch)
+            got = out.download(row_blocks * rows, offset=row_blocks * by)
+            assert np.array_equal(got, want), (fmt, by)
+    src.free()
+    out.free()
