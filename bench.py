@@ -422,7 +422,7 @@ def main() -> int:
         "config": {"workload": f"plain encode, {im.name} RGBA8 stride {im.pitch} B, waves+noise12 seed "
                                f"{im.seed:#x}; step = DXT1 + ETC1S launch (two streams), inputs HBM-resident, "
                                "4-buffer rotation",
-                   "global_batch": 1, "formats": ["DXT1", "ETC1S"], "parallelism": "single GPU"},
+                   "image": f"{im.width}x{im.height}", "formats": ["DXT1", "ETC1S"], "parallelism": "single GPU"},
         "roofline": roofline, "formats": formats, "parity": parity,
     }
     if not args.skip_stress:
