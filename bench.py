@@ -221,6 +221,60 @@ def bench_batch(g, hip, W, reps: int = 7):
     return res
 
 
+# ---- the kernels either side of the encoder ------------------------------------------------------
+
+def bench_around(g, hip, W, reps: int = 20):
+    """Decoders, the harness's RGB565 quality floor and the quality sums on the
+    8192^2 image: median time between events recorded on the kernels' stream
+    (decoders, floor: one launch each) or around the whole call (quality:
+    zeroing kernel, sum kernel, read-back), and GB/s of algorithmic bytes --
+    decode 8 B in + 64 B out per block, floor 4 B in + 4 B out per pixel,
+    round-trip quality 8 + 64 B read per block, image quality 8 B read per pixel."""
+    im = W.IMAGE_8192
+    stream = hip.Stream()
+    src = hip.DeviceBuffer(im.pitch * im.height)
+    g.synth_device(im.kind, im.seed, im.texture_id, im.noise, src, im.width, im.height, im.pitch, stream)
+    blocks = hip.DeviceBuffer(im.blocks * 8)
+    img = hip.DeviceBuffer(im.pitch * im.height)
+    res = {"workload": f"{im.name} RGBA8"}
+
+    def timed(fn, n=reps):
+        ts_ = []
+        for _ in range(n):
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            ts_.append(e0.elapsed_ms(e1))
+        return float(np.median(ts_))
+
+    def entry(ms, nbytes):
+        return {"ms": round(ms, 4), "gbps": round(nbytes / (ms / 1e3) / 1e9, 1),
+                "mps": round(im.pixels / 1e6 / (ms / 1e3), 1)}
+
+    for f, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+        g.compress_device(f, blocks, src, im.width, im.height, im.pitch, stream)
+        g.decode_device(f, img, blocks, im.width, im.height, im.pitch, stream)
+        res[f"decode_{key}"] = entry(timed(lambda: g.decode_device(f, img, blocks, im.width, im.height, im.pitch,
+                                                                   stream)), im.blocks * 72)
+        q = {}
+        ms = timed(lambda: q.update(g.quality_device(f, blocks, src, im.width, im.height, im.pitch, stream)), 5)
+        res[f"quality_{key}"] = {**entry(ms, im.blocks * 72), "psnr_rgb": round(q["psnr_rgb"], 3),
+                                 "psnr_y": round(q["psnr_y"], 3)}
+    g.rgb565_baseline_device(img, src, im.width, im.height, im.pitch, im.pitch, stream)
+    res["rgb565_floor"] = entry(timed(lambda: g.rgb565_baseline_device(img, src, im.width, im.height, im.pitch,
+                                                                       im.pitch, stream)), im.pixels * 8)
+    q = {}
+    ms = timed(lambda: q.update(g.quality_images_device(src, img, im.width, im.height, im.pitch, im.pitch, stream)),
+               5)
+    res["quality_floor"] = {**entry(ms, im.pixels * 8), "psnr_rgb": round(q["psnr_rgb"], 3)}
+    for x in (src, blocks, img):
+        x.free()
+    stream.destroy()
+    return res
+
+
 # ---- drop-in with host buffers (PCIe-inclusive) -------------------------------------------------
 
 def bench_dropin(g, W, reps: int = 5):
@@ -365,6 +419,7 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-dropin", action="store_true")
     ap.add_argument("--skip-stress", action="store_true")
+    ap.add_argument("--skip-around", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--profile", type=int, default=0, help="rocprof mode: N launches per kernel, no JSON")
     ap.add_argument("--sweep", action="store_true", help="launch-shape sweep table, no JSON")
@@ -428,6 +483,9 @@ def main() -> int:
     if not args.skip_stress:
         log("stress 16384^2")
         result["stress"] = bench_stress(g, hip, W)
+    if not args.skip_around:
+        log("decoders, quality floor, quality")
+        result["around"] = bench_around(g, hip, W)
     if not args.skip_batch:
         result["batched"] = bench_batch(g, hip, W)
     if not args.skip_serve:

@@ -229,6 +229,38 @@ def quality_device(fmt: int, blocks, source, width: int, height: int, pitch: int
     return q.as_dict()
 
 
+_sig("goofy_hip_quality_images_device", ctypes.c_int, _vp, _sz, _vp, _sz, _uint, _uint, _vp,
+     ctypes.POINTER(Quality))
+_sig("goofy_hip_rgb565_baseline_device", ctypes.c_int, _vp, _sz, _vp, _sz, _uint, _uint, _vp)
+_sig("goofy_hip_rgb565_baseline_device_timed", ctypes.c_int, _vp, _sz, _vp, _sz, _uint, _uint, _vp, _vp, _vp)
+
+
+def quality_images_device(a, b, width: int, height: int, pitch_a: int | None = None, pitch_b: int | None = None,
+                          stream=None) -> dict:
+    """getMsePsnr figures (Src/main.cpp:403-469) of two device RGBA8 images."""
+    pitch_a = width * 4 if pitch_a is None else pitch_a
+    pitch_b = width * 4 if pitch_b is None else pitch_b
+    q = Quality()
+    _check(_lib.goofy_hip_quality_images_device(_ptr(a), pitch_a, _ptr(b), pitch_b, width, height, _stream(stream),
+                                                ctypes.byref(q)))
+    return q.as_dict()
+
+
+def rgb565_baseline_device(dst, src, width: int, height: int, dst_pitch: int | None = None,
+                           src_pitch: int | None = None, stream=None, start_event=None, stop_event=None) -> None:
+    """The harness's quality floor, "Baseline (Downsample x2)" in RGB565
+    (generateDownsamlpedRgb565Test, Src/main.cpp:501-555), on the GPU."""
+    dst_pitch = width * 4 if dst_pitch is None else dst_pitch
+    src_pitch = width * 4 if src_pitch is None else src_pitch
+    if start_event is None and stop_event is None:
+        _check(_lib.goofy_hip_rgb565_baseline_device(_ptr(dst), dst_pitch, _ptr(src), src_pitch, width, height,
+                                                     _stream(stream)))
+    else:
+        _check(_lib.goofy_hip_rgb565_baseline_device_timed(
+            _ptr(dst), dst_pitch, _ptr(src), src_pitch, width, height, _stream(stream),
+            start_event.handle if start_event else None, stop_event.handle if stop_event else None))
+
+
 _sig("goofy_hip_traffic_probe", ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp, _vp, _vp)
 
 
@@ -326,5 +358,5 @@ __all__ = [
     "DXT1", "ETC1", "FORMATS", "GoofyError", "LaunchConfig", "TextureDesc", "CompressFunc", "Batch",
     "compressDXT1", "compressETC1", "compress", "compress_device", "compress_dxt1_fn", "compress_etc1_fn",
     "synth_host", "synth_device", "last_error", "version", "device_count", "WAVES", "STRESS",
-    "decode_device", "quality_device", "compress_device_timed",
+    "decode_device", "quality_device", "compress_device_timed", "quality_images_device", "rgb565_baseline_device",
 ]

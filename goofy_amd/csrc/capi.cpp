@@ -402,14 +402,12 @@ static double psnr_of(double mse, double peak)
     return 10.0 * std::log10((peak * peak) / mse);
 }
 
-int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_source, unsigned int width,
-                             unsigned int height, size_t pitch, void* stream, goofy_quality* out)
+// The quality sums live in a 64-byte device buffer: 4 x uint64 byte sums,
+// then the double luma sum.  Reads them back on `stream` and fills `out` with
+// getMsePsnr's figures (Src/main.cpp:403-469).
+extern "C++" template <typename Launch>
+static int run_quality(uint64_t pixels, hipStream_t stream, goofy_quality* out, Launch&& launch)
 {
-    clear_error();
-    if (!out) return fail(GOOFY_ERR_INVALID, "null output");
-    int rc = check_image_device(format, d_source, d_blocks, width, height, pitch);
-    if (rc != GOOFY_OK) return rc;
-    if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
     void* d = nullptr;
     hipError_t e = hipMalloc(&d, 64);
     if (e != hipSuccess) return fail_hip(e, "hipMalloc(quality sums)");
@@ -419,14 +417,14 @@ int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_sou
         void* p;
         ~Free() { (void)hipFree(p); }
     } guard{d};
-    e = launch_sqerr(format, d_blocks, d_source, pitch, width, height, sums, sum_y, (hipStream_t)stream);
+    e = launch(sums, sum_y);
     if (e != hipSuccess) return fail_hip(e, "quality kernel launch");
     unsigned long long h[5];
-    e = hipMemcpyAsync(h, d, 40, hipMemcpyDeviceToHost, (hipStream_t)stream);
-    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    e = hipMemcpyAsync(h, d, 40, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return fail_hip(e, "quality readback");
     goofy_quality q{};
-    q.pixels = (uint64_t)width * height;
+    q.pixels = pixels;
     for (int i = 0; i < 4; ++i) q.sse_byte[i] = h[i];
     std::memcpy(&q.sse_luma, &h[4], sizeof(double));
     const double n = (double)q.pixels;
@@ -446,6 +444,72 @@ int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_sou
     q.psnr_rgb = psnr_of(q.mse_rgb, 768.0);
     q.psnr_y = psnr_of(q.mse_y, 255.0);
     *out = q;
+    return GOOFY_OK;
+}
+
+int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_source, unsigned int width,
+                             unsigned int height, size_t pitch, void* stream, goofy_quality* out)
+{
+    clear_error();
+    if (!out) return fail(GOOFY_ERR_INVALID, "null output");
+    int rc = check_image_device(format, d_source, d_blocks, width, height, pitch);
+    if (rc != GOOFY_OK) return rc;
+    if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
+    return run_quality((uint64_t)width * height, (hipStream_t)stream, out, [&](unsigned long long* sums, double* sy) {
+        return launch_sqerr(format, d_blocks, d_source, pitch, width, height, sums, sy, (hipStream_t)stream);
+    });
+}
+
+// Checks shared by the two image-to-image entry points: width % 4 (one lane
+// per 4 pixels), 16-byte aligned rows, and sizes the kernels' index
+// arithmetic covers.
+static int check_rgba_pair(const void* d_a, size_t pitch_a, const void* d_b, size_t pitch_b, unsigned width,
+                           unsigned height, unsigned height_multiple)
+{
+    if (width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
+    if (height % height_multiple != 0)
+        return fail(GOOFY_ERR_HEIGHT, height_multiple == 2 ? "height must be even" : "bad height");
+    if (width == 0 || height == 0) return GOOFY_OK;
+    if (!d_a || !d_b) return fail(GOOFY_ERR_INVALID, "null buffer");
+    if (pitch_a < (size_t)width * 4 || pitch_a % 16 != 0 || pitch_b < (size_t)width * 4 || pitch_b % 16 != 0)
+        return fail(GOOFY_ERR_INVALID, "pitches must be >= 4*width and multiples of 16");
+    if ((uintptr_t)d_a % 16 != 0 || (uintptr_t)d_b % 16 != 0)
+        return fail(GOOFY_ERR_INVALID, "images must be 16-byte aligned");
+    const uint64_t qw = width / 4, n = qw * height;
+    if (!magic_ok(qw, n) || n > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
+    return GOOFY_OK;
+}
+
+int goofy_hip_quality_images_device(const void* d_a, size_t pitch_a, const void* d_b, size_t pitch_b,
+                                    unsigned int width, unsigned int height, void* stream, goofy_quality* out)
+{
+    clear_error();
+    if (!out) return fail(GOOFY_ERR_INVALID, "null output");
+    int rc = check_rgba_pair(d_a, pitch_a, d_b, pitch_b, width, height, 1);
+    if (rc != GOOFY_OK) return rc;
+    if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
+    return run_quality((uint64_t)width * height, (hipStream_t)stream, out, [&](unsigned long long* sums, double* sy) {
+        return launch_sqerr_images(d_a, pitch_a, d_b, pitch_b, width, height, sums, sy, (hipStream_t)stream);
+    });
+}
+
+int goofy_hip_rgb565_baseline_device(void* d_dst, size_t dst_pitch, const void* d_src, size_t src_pitch,
+                                     unsigned int width, unsigned int height, void* stream)
+{
+    return goofy_hip_rgb565_baseline_device_timed(d_dst, dst_pitch, d_src, src_pitch, width, height, stream, nullptr,
+                                                  nullptr);
+}
+
+int goofy_hip_rgb565_baseline_device_timed(void* d_dst, size_t dst_pitch, const void* d_src, size_t src_pitch,
+                                           unsigned int width, unsigned int height, void* stream, void* start_event,
+                                           void* stop_event)
+{
+    clear_error();
+    int rc = check_rgba_pair(d_dst, dst_pitch, d_src, src_pitch, width, height, 2);
+    if (rc != GOOFY_OK || width == 0 || height == 0) return rc;
+    hipError_t e = launch_rgb565_baseline(d_src, src_pitch, d_dst, dst_pitch, width, height, (hipStream_t)stream,
+                                          (hipEvent_t)start_event, (hipEvent_t)stop_event);
+    if (e != hipSuccess) return fail_hip(e, "rgb565 baseline kernel launch");
     return GOOFY_OK;
 }
 

@@ -16,6 +16,12 @@
 // Layout as in kernels.hip: one lane per 4x4 block, a wave covers 64
 // consecutive blocks; 8-byte block loads and 16-byte pixel-row accesses are
 // coalesced per wave.
+//
+// Also here: the harness's quality floor, a 2x-downsampled RGB565 copy of the
+// source (generateDownsamlpedRgb565Test, Src/main.cpp:501-555), and the
+// image-against-image form of the quality sums, so that a caller gets the
+// floor's PSNR and the encoder's PSNR over it, as the harness prints them.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -145,6 +151,44 @@ __global__ __launch_bounds__(256) void k_decode(const uint2* __restrict__ blocks
     }
 }
 
+// Squared error of one pixel pair into e[0..3] (bytes 0..3) and ey (luma
+// with the harness's weights, 0.21 / 0.72 / 0.07 on bytes 2 / 1 / 0; products
+// and sums kept unfused so each pixel's term is the one the harness forms).
+__device__ __forceinline__ void accum_pixel(uint32_t a, uint32_t d, uint32_t (&e)[4], double& ey)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) {
+        const int32_t diff = (int32_t)((a >> (8 * ch)) & 0xFFu) - (int32_t)((d >> (8 * ch)) & 0xFFu);
+        e[ch] += (uint32_t)(diff * diff);
+    }
+    const double y1 =
+        0.21 * (double)((a >> 16) & 0xFFu) + 0.72 * (double)((a >> 8) & 0xFFu) + 0.07 * (double)(a & 0xFFu);
+    const double y2 =
+        0.21 * (double)((d >> 16) & 0xFFu) + 0.72 * (double)((d >> 8) & 0xFFu) + 0.07 * (double)(d & 0xFFu);
+    const double dy = y2 - y1;
+    ey += dy * dy;
+}
+
+// Wave reduction, then one atomic per wave and sum.  Every lane of the wave
+// must reach this (no early return before it).
+__device__ __forceinline__ void flush_sums(const uint32_t (&e)[4], double ey, unsigned long long* sums,
+                                           double* sums_y)
+{
+    unsigned long long w[4] = {e[0], e[1], e[2], e[3]};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) w[ch] += __shfl_xor(w[ch], off, 64);
+        ey += __shfl_xor(ey, off, 64);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) atomicAdd(&sums[ch], w[ch]);
+        atomicAdd(sums_y, ey);
+    }
+}
+
 // Round-trip error: decode each block in registers and compare it with the
 // source pixels.  sums[0..3] = squared error of bytes 0..3 of the pixels
 // (uint64), sums_y = sum of squared luma differences with the harness's
@@ -169,35 +213,76 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint2* __restrict__ blocks,
             const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
             const uint32_t s[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const uint32_t a = s[c], d = px[4 * r + c];
-#pragma unroll
-                for (int ch = 0; ch < 4; ++ch) {
-                    const int32_t diff = (int32_t)((a >> (8 * ch)) & 0xFFu) - (int32_t)((d >> (8 * ch)) & 0xFFu);
-                    e[ch] += (uint32_t)(diff * diff);
-                }
-                const double y1 = 0.21 * (double)((a >> 16) & 0xFFu) + 0.72 * (double)((a >> 8) & 0xFFu) +
-                                  0.07 * (double)(a & 0xFFu);
-                const double y2 = 0.21 * (double)((d >> 16) & 0xFFu) + 0.72 * (double)((d >> 8) & 0xFFu) +
-                                  0.07 * (double)(d & 0xFFu);
-                const double dy = y2 - y1;
-                ey += dy * dy;
-            }
+            for (int c = 0; c < 4; ++c) accum_pixel(s[c], px[4 * r + c], e, ey);
         }
     }
-    // wave reduction, one atomic per wave and sum
-    unsigned long long w[4] = {e[0], e[1], e[2], e[3]};
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int ch = 0; ch < 4; ++ch) w[ch] += __shfl_xor(w[ch], off, 64);
-        ey += __shfl_xor(ey, off, 64);
+    flush_sums(e, ey, sums, sums_y);
+}
+
+// Squared errors of two whole RGBA8 images (getMsePsnr over a source and any
+// other image of its size).  One lane per 4 pixels of one row.
+__global__ __launch_bounds__(256) void k_sqerr_images(const uint8_t* __restrict__ a, uint64_t pitch_a,
+                                                      const uint8_t* __restrict__ b, uint64_t pitch_b, uint32_t qw,
+                                                      uint32_t nquads, uint64_t magic,
+                                                      unsigned long long* __restrict__ sums,
+                                                      double* __restrict__ sums_y)
+{
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t e[4] = {0, 0, 0, 0};
+    double ey = 0.0;
+    if (i < nquads) {
+        const uint32_t y = (uint32_t)(((uint64_t)i * magic) >> 40), x = i - y * qw;
+        const u32x4 va =
+            __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a + (uint64_t)y * pitch_a + (uint64_t)x * 16u));
+        const u32x4 vb =
+            __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(b + (uint64_t)y * pitch_b + (uint64_t)x * 16u));
+        accum_pixel(va.x, vb.x, e, ey);
+        accum_pixel(va.y, vb.y, e, ey);
+        accum_pixel(va.z, vb.z, e, ey);
+        accum_pixel(va.w, vb.w, e, ey);
     }
-    if ((threadIdx.x & 63u) == 0) {
-#pragma unroll
-        for (int ch = 0; ch < 4; ++ch) atomicAdd(&sums[ch], w[ch]);
-        atomicAdd(sums_y, ey);
-    }
+    flush_sums(e, ey, sums, sums_y);
+}
+
+// One 2x2 cell of the quality floor (Src/main.cpp:507-553).  p and q are the
+// cell's two pixels of its upper row, u and v those of its lower row, each
+// read as a little-endian word (byte 0 = "b", 1 = "g", 2 = "r").  Each
+// channel is averaged with rounding, (sum + 2) >> 2, cut to 5 ("r") or 6 bits
+// ("g", "b") and widened back to 8 bits as the harness does it: r5 << 3 | r5 >> 2,
+// g6 << 2 | g6 >> 4, and for "b" b6 << 2 | (b6 >> 2) & 7, three bits ORed into
+// a two-bit gap (so "b" is not a plain 6-bit replication).  Alpha is 255.
+__device__ __forceinline__ uint32_t rgb565_cell(uint32_t p, uint32_t q, uint32_t u, uint32_t v)
+{
+    const uint32_t m = 0x00FF00FFu;
+    const uint32_t s02 = (p & m) + (q & m) + (u & m) + (v & m) + 0x00020002u;  // bytes 0 and 2, 16-bit lanes
+    const uint32_t s1 = ((p >> 8) & 0xFFu) + ((q >> 8) & 0xFFu) + ((u >> 8) & 0xFFu) + ((v >> 8) & 0xFFu) + 2u;
+    const uint32_t b6 = ((s02 >> 2) & 0xFFu) >> 2;
+    const uint32_t r5 = ((s02 >> 18) & 0xFFu) >> 3;
+    const uint32_t g6 = ((s1 >> 2) & 0xFFu) >> 2;
+    const uint32_t r = (r5 << 3) | ((r5 >> 2) & 7u);
+    const uint32_t g = (g6 << 2) | ((g6 >> 4) & 3u);
+    const uint32_t b = (b6 << 2) | ((b6 >> 2) & 7u);
+    return (r << 16) | (g << 8) | b | 0xFF000000u;
+}
+
+// One lane per 4 pixels x 2 rows (two cells): 32 bytes in, 32 bytes out.
+__global__ __launch_bounds__(256) void k_rgb565_baseline(const uint8_t* __restrict__ src, uint64_t src_pitch,
+                                                         uint8_t* __restrict__ dst, uint64_t dst_pitch, uint32_t qw,
+                                                         uint32_t nunits, uint64_t magic)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nunits) return;
+    const uint32_t ry = (uint32_t)(((uint64_t)i * magic) >> 40), x = i - ry * qw;
+    const uint8_t* s0 = src + (uint64_t)(2u * ry) * src_pitch + (uint64_t)x * 16u;
+    const u32x4 t = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(s0));
+    const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(s0 + src_pitch));
+    const uint32_t c0 = rgb565_cell(t.x, t.y, w.x, w.y);
+    const uint32_t c1 = rgb565_cell(t.z, t.w, w.z, w.w);
+    const u32x4 o = {c0, c0, c1, c1};
+    uint8_t* d0 = dst + (uint64_t)(2u * ry) * dst_pitch + (uint64_t)x * 16u;
+    __builtin_nontemporal_store(o, reinterpret_cast<u32x4*>(d0));
+    __builtin_nontemporal_store(o, reinterpret_cast<u32x4*>(d0 + dst_pitch));
 }
 
 __global__ void k_zero_sums(unsigned long long* sums, double* sums_y)
@@ -238,6 +323,32 @@ hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pit
     else
         hipLaunchKernelGGL(k_sqerr<1>, grid, block, 0, stream, (const uint2*)blocks, (const uint8_t*)src,
                            (uint64_t)pitch, bw, nblocks, magic, d_sums, d_sum_y);
+    return hipGetLastError();
+}
+
+hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
+                               uint32_t height, unsigned long long* d_sums, double* d_sum_y, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_sums, dim3(1), dim3(64), 0, stream, d_sums, d_sum_y);
+    const uint32_t qw = width / 4, n = qw * height;
+    if (n == 0) return hipGetLastError();
+    hipLaunchKernelGGL(k_sqerr_images, dim3((n + 255u) / 256u), dim3(256), 0, stream, (const uint8_t*)a,
+                       (uint64_t)pitch_a, (const uint8_t*)b, (uint64_t)pitch_b, qw, n, div_magic(qw), d_sums, d_sum_y);
+    return hipGetLastError();
+}
+
+hipError_t launch_rgb565_baseline(const void* src, size_t src_pitch, void* dst, size_t dst_pitch, uint32_t width,
+                                  uint32_t height, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
+{
+    const uint32_t qw = width / 4, n = qw * (height / 2);
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + 255u) / 256u), block(256);
+    if (start || stop)
+        hipExtLaunchKernelGGL(k_rgb565_baseline, grid, block, 0, stream, start, stop, 0, (const uint8_t*)src,
+                              (uint64_t)src_pitch, (uint8_t*)dst, (uint64_t)dst_pitch, qw, n, div_magic(qw));
+    else
+        hipLaunchKernelGGL(k_rgb565_baseline, grid, block, 0, stream, (const uint8_t*)src, (uint64_t)src_pitch,
+                           (uint8_t*)dst, (uint64_t)dst_pitch, qw, n, div_magic(qw));
     return hipGetLastError();
 }
 

@@ -71,6 +71,11 @@ hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width,
                          hipStream_t stream);
 hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
                         unsigned long long* d_sums, double* d_sum_y, hipStream_t stream);
+hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
+                               uint32_t height, unsigned long long* d_sums, double* d_sum_y, hipStream_t stream);
+hipError_t launch_rgb565_baseline(const void* src, size_t src_pitch, void* dst, size_t dst_pitch, uint32_t width,
+                                  uint32_t height, hipStream_t stream, hipEvent_t start = nullptr,
+                                  hipEvent_t stop = nullptr);
 
 // Error reporting: every C-ABI function stores a message for its thread.
 int fail(int code, const std::string& msg);

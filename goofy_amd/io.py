@@ -37,7 +37,13 @@ def load_png_rgba8(path: str | Path) -> tuple[np.ndarray, int, int]:
     from PIL import Image  # PNG decoding only; the encoder never sees PIL objects
 
     with Image.open(path) as im:
-        rgba = np.asarray(im.convert("RGBA"), dtype=np.uint8)
+        if im.mode.startswith("I"):
+            # 16-bit greyscale: the harness's decoder (lodepng, to 8-bit RGBA)
+            # keeps the high byte of each sample; PIL's RGBA conversion would clip
+            g = (np.asarray(im).astype(np.uint32) >> 8).astype(np.uint8)
+            rgba = np.stack([g, g, g, np.full_like(g, 255)], axis=-1)
+        else:
+            rgba = np.asarray(im.convert("RGBA"), dtype=np.uint8)
     h, w = rgba.shape[:2]
     if w % 16 != 0:
         raise ImageRejected(f"{path}: width {w} is not a multiple of 16")

@@ -117,6 +117,29 @@ typedef struct goofy_quality {
 GOOFY_HIP_API int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_source, unsigned int width,
                                            unsigned int height, size_t pitch, void* stream, goofy_quality* out);
 
+/* getMsePsnr (Src/main.cpp:403-469) between two device RGBA8 images of the
+ * same size, e.g. a source and its quality floor below: same figures and
+ * labels as goofy_hip_quality_device.  width % 4 == 0; rows 16-byte aligned.
+ * Synchronises `stream`. */
+GOOFY_HIP_API int goofy_hip_quality_images_device(const void* d_a, size_t pitch_a, const void* d_b, size_t pitch_b,
+                                                  unsigned int width, unsigned int height, void* stream,
+                                                  goofy_quality* out);
+
+/* The harness's quality floor, "Baseline (Downsample x2)" in RGB565
+ * (generateDownsamlpedRgb565Test, Src/main.cpp:501-555, called at :900):
+ * every 2x2 cell of the source becomes one colour, its rounded box average
+ * cut to 5/6/6 bits and widened back to 8 bits as the harness does it,
+ * alpha 255; the output has the source's size.  Bit-exact with the harness.
+ * width % 4 == 0 (the harness needs even sizes; its images have width % 16),
+ * height % 2 == 0; pitches >= 4*width and multiples of 16; 16-byte aligned.
+ * Returns after the launch.  The _timed form records the two events around
+ * the kernel's own dispatch, as goofy_hip_compress_device_timed. */
+GOOFY_HIP_API int goofy_hip_rgb565_baseline_device(void* d_dst, size_t dst_pitch, const void* d_src, size_t src_pitch,
+                                                   unsigned int width, unsigned int height, void* stream);
+GOOFY_HIP_API int goofy_hip_rgb565_baseline_device_timed(void* d_dst, size_t dst_pitch, const void* d_src,
+                                                         size_t src_pitch, unsigned int width, unsigned int height,
+                                                         void* stream, void* start_event, void* stop_event);
+
 /* Roofline calibration: the plain encode kernel's memory traffic (same
  * addresses, tiles and streaming hints; 64 bytes read and 8 written per
  * block) with no encode arithmetic.  Its time is the floor for the encode

@@ -12,7 +12,8 @@
  *
  * Used for round-trip invariants on the encoder (every decoded colour lies in
  * the palette the encoder picked) and as the oracle of the GPU decoders.
- * Also holds the multi-threaded striped driver of the encoder oracle.
+ * Also holds the harness's RGB565 quality floor and the multi-threaded
+ * striped driver of the encoder oracle.
  */
 #include <pthread.h>
 #include <stddef.h>
@@ -121,6 +122,32 @@ typedef struct {
     unsigned width, height, row0, row1;
     size_t stride;
 } stripe_job;
+
+/* Src/main.cpp:501-555, written per channel from its arithmetic: a pixel is
+ * a little-endian word whose byte 2 the harness calls "r", byte 1 "g" and
+ * byte 0 "b".  Box average (sum of 4 + 2) / 4; "r" keeps 5 bits and is
+ * widened r5*8 + r5/4; "g" keeps 6 bits, g6*4 + g6/16; "b" keeps 6 bits but
+ * is widened like a 5-bit value, b6*4 OR (b6/4 mod 8), as the harness has it. */
+void goofy_oracle_rgb565_baseline(uint8_t* dst, size_t dst_pitch, const uint8_t* src, size_t src_pitch,
+                                  unsigned width, unsigned height)
+{
+    for (unsigned y = 0; y + 1 < height; y += 2) {
+        for (unsigned x = 0; x + 1 < width; x += 2) {
+            unsigned avg[3];
+            for (int c = 0; c < 3; ++c) {
+                const unsigned sum = src[y * src_pitch + 4 * x + c] + src[y * src_pitch + 4 * (x + 1) + c] +
+                                     src[(y + 1) * src_pitch + 4 * x + c] +
+                                     src[(y + 1) * src_pitch + 4 * (x + 1) + c];
+                avg[c] = (sum + 2) / 4;
+            }
+            const unsigned b6 = avg[0] / 4, g6 = avg[1] / 4, r5 = avg[2] / 8;
+            const uint8_t out[4] = {(uint8_t)((b6 * 4) | ((b6 / 4) % 8)), (uint8_t)(g6 * 4 + g6 / 16),
+                                    (uint8_t)(r5 * 8 + r5 / 4), 255};
+            for (unsigned dy = 0; dy < 2; ++dy)
+                for (unsigned dx = 0; dx < 2; ++dx) memcpy(dst + (y + dy) * dst_pitch + 4 * (x + dx), out, 4);
+        }
+    }
+}
 
 static void* stripe_main(void* arg)
 {

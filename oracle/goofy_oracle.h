@@ -47,6 +47,12 @@ void goofy_oracle_decode_etc1(const uint8_t* block, uint8_t* out);
 int goofy_oracle_decode_image(int fmt, uint8_t* dst, const uint8_t* blocks, unsigned width,
                               unsigned height);
 
+/* The benchmark harness's quality floor (generateDownsamlpedRgb565Test,
+ * Src/main.cpp:501-555): each 2x2 cell -> rounded box average, cut to 5/6/6
+ * bits and widened back as the harness does; width and height even. */
+void goofy_oracle_rgb565_baseline(uint8_t* dst, size_t dst_pitch, const uint8_t* src, size_t src_pitch,
+                                  unsigned width, unsigned height);
+
 #ifdef __cplusplus
 }
 #endif

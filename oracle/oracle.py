@@ -5,6 +5,9 @@
 * ``Reference`` -- the reference header compiled from its own source into
   oracle/_ref/libgoofy_ref.so (present when the reference tree was available
   at build time; the built file travels with the repository snapshot).
+* ``Harness`` -- the reference benchmark harness (Src/main.cpp) compiled the
+  same way into oracle/_ref/libgoofy_harness.so: its PNG loader, DDS/KTX
+  writers, image decoders, getMsePsnr and the RGB565 quality floor.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import
 this module, and only as the checker.
@@ -20,6 +23,11 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "libgoofy_oracle.so"
 REF_SO = HERE / "_ref" / "libgoofy_ref.so"
+HARNESS_SO = HERE / "_ref" / "libgoofy_harness.so"
+
+# MsePsnr field order (Src/main.cpp:367-385)
+MSE_PSNR_FIELDS = ("mse_r", "mse_g", "mse_b", "mse_a", "mse_max", "mse_rgb", "mse_y",
+                   "psnr_r", "psnr_g", "psnr_b", "psnr_a", "psnr_min", "psnr_rgb", "psnr_y")
 
 DXT1, ETC1 = 0, 1
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -60,7 +68,17 @@ class Oracle:
         lib.goofy_oracle_decode_bc1.argtypes = [_u8p, _u8p]
         lib.goofy_oracle_decode_etc1.argtypes = [_u8p, _u8p]
         lib.goofy_oracle_decode_image.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.c_uint, ctypes.c_uint]
+        lib.goofy_oracle_rgb565_baseline.argtypes = [_u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t, ctypes.c_uint,
+                                                     ctypes.c_uint]
         self.lib = lib
+
+    def rgb565_baseline(self, img: np.ndarray, width: int, height: int, pitch: int | None = None) -> np.ndarray:
+        """The harness's quality floor; returns a tight width*height*4 image."""
+        pitch = pitch or width * 4
+        out = np.zeros(width * height * 4, dtype=np.uint8)
+        self.lib.goofy_oracle_rgb565_baseline(_ptr(out), width * 4, _ptr(np.ascontiguousarray(img)), pitch, width,
+                                              height)
+        return out
 
     # -- blocks ---------------------------------------------------------------
     def encode_blocks(self, fmt: int, blocks: np.ndarray) -> np.ndarray:
@@ -165,5 +183,65 @@ class Reference:
         return out
 
 
+class Harness:
+    """The reference benchmark harness (Src/main.cpp) compiled from its own
+    source; see oracle/harness_shim.cpp."""
+
+    def __init__(self, path: Path = HARNESS_SO):
+        if not path.exists():
+            raise FileNotFoundError(f"{path} not built (reference tree absent at build time)")
+        lib = ctypes.CDLL(str(path))
+        uintp = ctypes.POINTER(ctypes.c_uint)
+        lib.goofy_harness_load_png.argtypes = [ctypes.c_char_p, uintp, uintp]
+        lib.goofy_harness_load_png.restype = ctypes.c_void_p
+        lib.goofy_harness_free.argtypes = [ctypes.c_void_p]
+        lib.goofy_harness_mse_psnr.argtypes = [_u8p, _u8p, ctypes.c_uint, ctypes.c_uint,
+                                               ctypes.POINTER(ctypes.c_double)]
+        lib.goofy_harness_rgb565_baseline.argtypes = [_u8p, ctypes.c_uint, ctypes.c_uint, _u8p]
+        lib.goofy_harness_decompress.argtypes = [ctypes.c_int, _u8p, ctypes.c_uint, ctypes.c_uint, _u8p]
+        lib.goofy_harness_save.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, _u8p, ctypes.c_size_t,
+                                           ctypes.c_uint, ctypes.c_uint]
+        self.lib = lib
+
+    def load_png(self, path) -> tuple[np.ndarray, int, int] | None:
+        """loadPngAsRgba8: (RGBA8 copy, width, height), or None when refused."""
+        w, h = ctypes.c_uint(0), ctypes.c_uint(0)
+        p = self.lib.goofy_harness_load_png(str(path).encode(), ctypes.byref(w), ctypes.byref(h))
+        if not p:
+            return None
+        try:
+            n = w.value * h.value * 4
+            img = np.ctypeslib.as_array(ctypes.cast(p, _u8p), shape=(n,)).copy()
+        finally:
+            self.lib.goofy_harness_free(p)
+        return img, w.value, h.value
+
+    def mse_psnr(self, a: np.ndarray, b: np.ndarray, width: int, height: int) -> dict:
+        """getMsePsnr over two tight RGBA8 images."""
+        out = (ctypes.c_double * 14)()
+        self.lib.goofy_harness_mse_psnr(_ptr(np.ascontiguousarray(a)), _ptr(np.ascontiguousarray(b)), width, height,
+                                        out)
+        return dict(zip(MSE_PSNR_FIELDS, list(out)))
+
+    def rgb565_baseline(self, img: np.ndarray, width: int, height: int) -> np.ndarray:
+        out = np.zeros(width * height * 4, dtype=np.uint8)
+        self.lib.goofy_harness_rgb565_baseline(_ptr(np.ascontiguousarray(img)), width, height, _ptr(out))
+        return out
+
+    def decompress(self, fmt: int, blocks: np.ndarray, width: int, height: int) -> np.ndarray:
+        out = np.zeros(width * height * 4, dtype=np.uint8)
+        self.lib.goofy_harness_decompress(fmt, _ptr(np.ascontiguousarray(blocks)), width, height, _ptr(out))
+        return out
+
+    def save(self, container: int, fmt: int, path, data: np.ndarray, width: int, height: int) -> None:
+        """saveDds (container 0) / saveKtx (container 1) to `path`."""
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        self.lib.goofy_harness_save(container, fmt, str(path).encode(), _ptr(data), data.size, width, height)
+
+
 def have_reference() -> bool:
     return REF_SO.exists()
+
+
+def have_harness() -> bool:
+    return HARNESS_SO.exists()

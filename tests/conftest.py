@@ -30,6 +30,19 @@ def reference():
 
 
 @pytest.fixture(scope="session")
+def harness():
+    from oracle.oracle import Harness, have_harness
+    if not have_harness():
+        pytest.skip("oracle/_ref/libgoofy_harness.so (reference harness compiled from its own source) not built")
+    return Harness()
+
+
+@pytest.fixture(scope="session")
+def real_images():
+    return json.loads((GOLDEN / "real_images.json").read_text())
+
+
+@pytest.fixture(scope="session")
 def golden_blocks():
     z = np.load(GOLDEN / "blocks.npz")
     return {k: z[k] for k in z.files}

@@ -9,10 +9,13 @@ and records what it outputs:
   img512.npz   -- the full DXT1 / ETC1S outputs of the 512x512 waves image;
   real_images.json -- for three of the reference's own test images
                   (test-data/parrot_red.png, kodim23.png, patterns.png, copied
-                  under tests/golden/images/ as data): SHA-256 of the RGBA8
-                  input as the harness loads it and of both encodings, and the
-                  harness's PSNR figures of the round trip through the
-                  reference's decoders;
+                  under tests/golden/images/ as data), from the reference
+                  harness (Src/main.cpp, compiled into
+                  oracle/_ref/libgoofy_harness.so): SHA-256 of the RGBA8 input
+                  as its loader returns it and of both encodings; the
+                  harness's getMsePsnr figures of the round trip through its
+                  decoders; the SHA-256 of its RGB565 quality floor and that
+                  floor's getMsePsnr figures;
   digests.json -- SHA-256 of inputs and outputs of the 512^2, 8192^2 and the
                   pitch-padded 16384^2 stress images, and of a 16-texture
                   batch with widths that are multiples of 4 but not of 16
@@ -20,7 +23,7 @@ and records what it outputs:
                   16 pixels, then cropped block-wise).
 
 Inputs come from the library's seeded integer generator; the input digests
-pin that generator too.  Run: python tests/golden/make_golden.py
+pin that generator too.  Run: python tests/golden/make_golden.py [real]
 """
 from __future__ import annotations
 
@@ -36,7 +39,7 @@ sys.path.insert(0, str(ROOT))
 
 import goofy_amd  # noqa: E402
 from goofy_amd import workloads as W  # noqa: E402
-from oracle.oracle import Reference, aligned_empty  # noqa: E402
+from oracle.oracle import Harness, Reference, aligned_empty  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 
@@ -164,21 +167,28 @@ def main() -> int:
     (OUT / "digests.json").write_text(json.dumps(digests, indent=1) + "\n")
     print("digests.json written")
 
-    from goofy_amd.io import load_png_rgba8
+    write_real_images(ref)
+    return 0
+
+
+def write_real_images(ref: Reference) -> None:
+    harness = Harness()
     real = {}
     for name in ("parrot_red.png", "kodim23.png", "patterns.png"):
-        img, w, h = load_png_rgba8(OUT / "images" / name)
+        img, w, h = harness.load_png(OUT / "images" / name)
         entry = {"width": w, "height": h, "input_sha256": sha(img)}
         for fmt, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
             out = encode_image_ref(ref, fmt, img, w, h, w * 4)
             entry[f"{key}_sha256"] = sha(out)
-            dec = ref.decode_blocks(fmt, out.reshape(-1, 8))  # (n, 64): blocks row-major
-            dec = dec.reshape(h // 4, w // 4, 4, 16).transpose(0, 2, 1, 3).reshape(-1)
+            dec = harness.decompress(fmt, out, w, h)
             entry[f"{key}_psnr"] = harness_psnr(img, dec)
+            entry[f"{key}_quality"] = harness.mse_psnr(img, dec, w, h)
+        base = harness.rgb565_baseline(img, w, h)
+        entry["baseline_sha256"] = sha(base)
+        entry["baseline_quality"] = harness.mse_psnr(img, base, w, h)
         real[name] = entry
-        print(name, w, h, {k: v for k, v in entry.items() if k.endswith("psnr")})
+        print(name, w, h, {k: round(v["psnr_rgb"], 3) for k, v in entry.items() if k.endswith("quality")})
     (OUT / "real_images.json").write_text(json.dumps(real, indent=1) + "\n")
-    return 0
 
 
 def harness_psnr(src: np.ndarray, dec: np.ndarray) -> dict:
@@ -196,4 +206,7 @@ def harness_psnr(src: np.ndarray, dec: np.ndarray) -> dict:
 
 
 if __name__ == "__main__":
+    if sys.argv[1:] == ["real"]:  # only real_images.json
+        write_real_images(Reference())
+        sys.exit(0)
     sys.exit(main())

@@ -86,6 +86,27 @@ def test_device_api_validation_without_device():
     assert e.value.status == goofy_amd.ERR_INVALID
 
 
+def test_floor_and_image_quality_validation_without_device():
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.rgb565_baseline_device(16, 16, 6, 4)
+    assert e.value.status == goofy_amd.ERR_WIDTH
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.rgb565_baseline_device(16, 16, 8, 3)
+    assert e.value.status == goofy_amd.ERR_HEIGHT
+    for args in [(16, 16, 8, 4, 24, 32),   # dst pitch not a multiple of 16
+                 (16, 16, 8, 4, 32, 16),   # src pitch < 4*width
+                 (16, 24, 8, 4, 32, 32)]:  # misaligned source
+        with pytest.raises(goofy_amd.GoofyError) as e:
+            goofy_amd.rgb565_baseline_device(*args)
+        assert e.value.status == goofy_amd.ERR_INVALID, args
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.quality_images_device(16, 0, 8, 4)
+    assert e.value.status == goofy_amd.ERR_INVALID
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.quality_images_device(16, 16, 10, 4)
+    assert e.value.status == goofy_amd.ERR_WIDTH
+
+
 def test_batch_validation_without_device():
     bad = [dict(src=16, dst=8, width=6, height=4, pitch=32, format=0)]
     with pytest.raises(goofy_amd.GoofyError) as e:

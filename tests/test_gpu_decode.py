@@ -95,3 +95,71 @@ def test_quality_of_black_image_closed_form(gpu):
     assert q["sse_byte"] == [0, 4 * 256, 0, 0]
     assert q["psnr_g"] == pytest.approx(10 * math.log10(255.0 ** 2 / 4.0), abs=1e-12)
     assert q["psnr_min"] == q["psnr_g"] and q["psnr_a"] > 1e300 and q["psnr_r"] > 1e300
+
+
+# ---- the harness's RGB565 quality floor and image-to-image quality ------------------------------
+
+def test_rgb565_floor_vs_oracle_with_pitches(gpu, oracle):
+    from goofy_amd import hip
+    rng = np.random.default_rng(11)
+    w, h = 200, 34
+    sp, dp = w * 4 + 48, w * 4 + 32
+    src = rng.integers(0, 256, sp * h, dtype=np.uint8)
+    d_src = hip.DeviceBuffer.from_array(src)
+    d_dst = hip.DeviceBuffer(dp * h, fill=0x5A)
+    print(f"rgb565 floor {w}x{h} pitches {sp}/{dp}", flush=True)
+    gpu.rgb565_baseline_device(d_dst, d_src, w, h, dst_pitch=dp, src_pitch=sp)
+    hip.synchronize()
+    got = d_dst.download().reshape(h, dp)
+    want = oracle.rgb565_baseline(src, w, h, pitch=sp).reshape(h, w * 4)
+    assert np.array_equal(got[:, : w * 4], want)
+    assert (got[:, w * 4:] == 0x5A).all()
+
+
+def _check_quality(q, ref, n):
+    # squared errors are integers and the MSEs come from them exactly as in
+    # getMsePsnr; only the luma sum's order differs (relative 1e-9)
+    for k in ("mse_r", "mse_g", "mse_b", "mse_a", "mse_max", "mse_rgb",
+              "psnr_r", "psnr_g", "psnr_b", "psnr_a", "psnr_min", "psnr_rgb"):
+        assert q[k] == ref[k], k
+    assert q["pixels"] == n
+    assert q["mse_y"] == pytest.approx(ref["mse_y"], rel=1e-9)
+    assert q["psnr_y"] == pytest.approx(ref["psnr_y"], abs=1e-7)
+
+
+@pytest.mark.parametrize("name", ["parrot_red.png", "kodim23.png", "patterns.png"])
+def test_floor_and_quality_real_images_match_harness(gpu, real_images, name):
+    """GPU floor, its PSNR, and the encoder's round-trip PSNR on the
+    reference's test images against the harness's own figures (fixtures)."""
+    import hashlib
+
+    from conftest import GOLDEN
+    from goofy_amd import hip
+    from goofy_amd import io as gio
+    g = real_images[name]
+    img, w, h = gio.load_png_rgba8(GOLDEN / "images" / name)
+    assert hashlib.sha256(img.tobytes()).hexdigest() == g["input_sha256"]
+    print(f"floor+quality {name} {w}x{h}", flush=True)
+    d_src = hip.DeviceBuffer.from_array(img)
+    d_base = hip.DeviceBuffer(img.size)
+    gpu.rgb565_baseline_device(d_base, d_src, w, h)
+    hip.synchronize()
+    assert hashlib.sha256(d_base.download().tobytes()).hexdigest() == g["baseline_sha256"]
+    _check_quality(gpu.quality_images_device(d_src, d_base, w, h), g["baseline_quality"], w * h)
+    blocks = hip.DeviceBuffer((w // 4) * (h // 4) * 8)
+    for fmt, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+        gpu.compress_device(fmt, blocks, d_src, w, h, w * 4)
+        _check_quality(gpu.quality_device(fmt, blocks, d_src, w, h), g[f"{key}_quality"], w * h)
+
+
+def test_quality_images_closed_form(gpu):
+    from goofy_amd import hip
+    w, h = 64, 8
+    a = np.tile(np.array([10, 20, 30, 255], np.uint8), w * h)
+    b = a.copy()
+    b[1::4] += 2  # byte 1 ("g") off by 2 everywhere
+    q = gpu.quality_images_device(hip.DeviceBuffer.from_array(a), hip.DeviceBuffer.from_array(b), w, h)
+    assert q["sse_byte"] == [0, 4 * w * h, 0, 0]
+    assert q["mse_g"] == 4.0 and q["mse_rgb"] == 4.0
+    assert q["mse_y"] == pytest.approx((0.72 * 22 - 0.72 * 20) ** 2, rel=1e-12)
+    assert q["psnr_rgb"] == pytest.approx(10 * math.log10(768.0 ** 2 / 4.0), abs=1e-12)
