@@ -149,6 +149,16 @@ def bench_stress(g, hip, W, launches: int = 20):
         ms = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
         res[key] = {"kernel_ms_mean": round(ms, 4), "mps": round(im.pixels / 1e6 / (ms / 1e3), 1),
                     "gbps": round(algo / (ms / 1e3) / 1e9, 1), "bit_exact": exact}
+        # the launch-shape tuner on this shape, then the same timing with its choice
+        cfg, _ = g.tune_device(f, out, src, im.width, im.height, im.pitch, stream, reps=3)
+        ev = [(hip.Event(), hip.Event()) for _ in range(launches)]
+        for a, b in ev:
+            g.compress_device_timed(f, out, src, im.width, im.height, im.pitch, a, b, stream, config=cfg)
+        stream.synchronize()
+        ms = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
+        res[key]["tuned"] = {"config": cfg, "kernel_ms_mean": round(ms, 4),
+                             "mps": round(im.pixels / 1e6 / (ms / 1e3), 1), "gbps": round(algo / (ms / 1e3) / 1e9, 1),
+                             "bit_exact": sha(out.download()) == digests[f"{key}_sha256"]}
     src.free()
     out.free()
     stream.destroy()
@@ -380,32 +390,43 @@ def profile_mode(g, hip, W, launches: int):
 
 
 def sweep_mode(g, hip, W, launches: int = 40):
-    """Launch-shape sweep of the plain kernels on the 8192^2 image: median kernel
-    time (dispatch-recorded events) per blocks/lane, waves/workgroup, streaming."""
-    im = W.IMAGE_8192
-    srcs = [hip.DeviceBuffer(im.pitch * im.height) for _ in range(4)]
-    for s in srcs:
-        g.synth_device(im.kind, im.seed, im.texture_id, im.noise, s, im.width, im.height, im.pitch)
-    out = hip.DeviceBuffer(im.blocks * 8)
-    hip.synchronize()
-    algo = W.algorithmic_bytes(im.width, im.height)
-    print("| format | blocks/lane | waves/WG | streaming | median us | GB/s |")
-    print("|---|---|---|---|---|---|")
-    for fmt in (W.DXT1, W.ETC1):
-        for bpl in (1, 2, 4):
-            for wg in (1, 2, 4, 8):
-                for nt in (0, 1):
-                    cfg = dict(blocks_per_lane=bpl, wg_waves=wg, waves_per_cu=0, nontemporal=nt)
-                    for i in range(6):
-                        g.compress_device(fmt, out, srcs[i % 4], im.width, im.height, im.pitch, config=cfg)
-                    ev = [(hip.Event(), hip.Event()) for _ in range(launches)]
-                    for i, (a, b) in enumerate(ev):
-                        g.compress_device_timed(fmt, out, srcs[i % 4], im.width, im.height, im.pitch, a, b,
-                                                config=cfg)
-                    hip.synchronize()
-                    med = float(np.median([a.elapsed_ms(b) for a, b in ev]))
-                    print(f"| {'DXT1' if fmt == W.DXT1 else 'ETC1S'} | {bpl} | {wg} | {nt} | {med * 1e3:.2f} | "
-                          f"{algo / med / 1e6:.0f} |", flush=True)
+    """Launch-shape sweep of the plain kernels: median kernel time
+    (dispatch-recorded events) per blocks/lane, waves/workgroup, streaming and
+    tile order, on the 8192^2 image (four rotated inputs) and on the 16384^2
+    stress image with its 65792-byte pitch (streaming only)."""
+    def table(im, nbuf, nts):
+        srcs = [hip.DeviceBuffer(im.pitch * im.height) for _ in range(nbuf)]
+        for s in srcs:
+            g.synth_device(im.kind, im.seed, im.texture_id, im.noise, s, im.width, im.height, im.pitch)
+        out = hip.DeviceBuffer(im.blocks * 8)
+        hip.synchronize()
+        algo = W.algorithmic_bytes(im.width, im.height)
+        print(f"\n{im.name}, pitch {im.pitch} B, {nbuf} input(s)\n")
+        print("| format | blocks/lane | waves/WG | streaming | tile order | median us | GB/s |")
+        print("|---|---|---|---|---|---|---|")
+        for fmt in (W.DXT1, W.ETC1):
+            for bpl in (1, 2, 4):
+                for wg in (1, 2, 4, 8):
+                    for nt in nts:
+                        for order in (0, 1):
+                            cfg = dict(blocks_per_lane=bpl, wg_waves=wg, waves_per_cu=0, nontemporal=nt,
+                                       tile_order=order)
+                            for i in range(4):
+                                g.compress_device(fmt, out, srcs[i % nbuf], im.width, im.height, im.pitch,
+                                                  config=cfg)
+                            ev = [(hip.Event(), hip.Event()) for _ in range(launches)]
+                            for i, (a, b) in enumerate(ev):
+                                g.compress_device_timed(fmt, out, srcs[i % nbuf], im.width, im.height, im.pitch, a,
+                                                        b, config=cfg)
+                            hip.synchronize()
+                            med = float(np.median([a.elapsed_ms(b) for a, b in ev]))
+                            print(f"| {'DXT1' if fmt == W.DXT1 else 'ETC1S'} | {bpl} | {wg} | {nt} | {order} | "
+                                  f"{med * 1e3:.2f} | {algo / med / 1e6:.0f} |", flush=True)
+        for x in srcs + [out]:
+            x.free()
+
+    table(W.IMAGE_8192, 4, (0, 1))
+    table(W.STRESS_16384, 1, (1,))
 
 
 def main() -> int:

@@ -58,7 +58,7 @@ CompressFunc = ctypes.CFUNCTYPE(ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
 
 class LaunchConfig(ctypes.Structure):
     _fields_ = [("blocks_per_lane", ctypes.c_int), ("wg_waves", ctypes.c_int),
-                ("waves_per_cu", ctypes.c_int), ("nontemporal", ctypes.c_int)]
+                ("waves_per_cu", ctypes.c_int), ("nontemporal", ctypes.c_int), ("tile_order", ctypes.c_int)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -180,6 +180,19 @@ def compress_device(fmt: int, dst, src, width: int, height: int, pitch: int, str
         cfg = config if isinstance(config, LaunchConfig) else LaunchConfig(**config)
         _check(_lib.goofy_hip_compress_device_cfg(fmt, _ptr(dst), _ptr(src), width, height, pitch,
                                                   _stream(stream), ctypes.byref(cfg)))
+
+
+_sig("goofy_hip_tune_device", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp, ctypes.c_int,
+     ctypes.POINTER(LaunchConfig), ctypes.POINTER(ctypes.c_float))
+
+
+def tune_device(fmt: int, dst, src, width: int, height: int, pitch: int, stream=None, reps: int = 3) -> tuple:
+    """Fastest launch shape for this image shape: (config dict, median kernel ms)."""
+    cfg = LaunchConfig()
+    ms = ctypes.c_float()
+    _check(_lib.goofy_hip_tune_device(fmt, _ptr(dst), _ptr(src), width, height, pitch, _stream(stream), reps,
+                                      ctypes.byref(cfg), ctypes.byref(ms)))
+    return cfg.as_dict(), float(ms.value)
 
 
 def compress_device_timed(fmt: int, dst, src, width: int, height: int, pitch: int, start_event, stop_event,
@@ -359,4 +372,5 @@ __all__ = [
     "compressDXT1", "compressETC1", "compress", "compress_device", "compress_dxt1_fn", "compress_etc1_fn",
     "synth_host", "synth_device", "last_error", "version", "device_count", "WAVES", "STRESS",
     "decode_device", "quality_device", "compress_device_timed", "quality_images_device", "rgb565_baseline_device",
+    "tune_device",
 ]

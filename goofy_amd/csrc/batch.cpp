@@ -162,9 +162,9 @@ void load_file_cache()
     FILE* f = std::fopen(path, "r");
     if (!f) return;
     char arch[64];
-    int lb, lpt, mix, bpl, wg, wpc, nt;
-    while (std::fscanf(f, "%63s %d %d %d %d %d %d %d", arch, &lb, &lpt, &mix, &bpl, &wg, &wpc, &nt) == 8) {
-        LaunchCfg c{bpl, wg, wpc, nt};
+    int lb, lpt, mix, bpl, wg, wpc, nt, order;
+    while (std::fscanf(f, "%63s %d %d %d %d %d %d %d %d", arch, &lb, &lpt, &mix, &bpl, &wg, &wpc, &nt, &order) == 9) {
+        LaunchCfg c{bpl, wg, wpc, nt, order};
         if (cfg_valid(c)) tune_cache()[Key{arch, lb, lpt, mix}] = c;
     }
     std::fclose(f);
@@ -178,9 +178,9 @@ void save_file_cache()
     if (!f) return;
     for (const auto& kv : tune_cache()) {
         const LaunchCfg& c = kv.second;
-        std::fprintf(f, "%s %d %d %d %d %d %d %d\n", std::get<0>(kv.first).c_str(), std::get<1>(kv.first),
+        std::fprintf(f, "%s %d %d %d %d %d %d %d %d\n", std::get<0>(kv.first).c_str(), std::get<1>(kv.first),
                      std::get<2>(kv.first), std::get<3>(kv.first), c.blocks_per_lane, c.wg_waves, c.waves_per_cu,
-                     c.nontemporal);
+                     c.nontemporal, c.tile_order);
     }
     std::fclose(f);
 }
@@ -239,7 +239,7 @@ int goofy_batch_create(const goofy_texture_desc* textures, size_t count, const g
         b->cfg = it != tune_cache().end() ? it->second : default_batch_cfg();
     }
     if (ucfg) {
-        LaunchCfg c{ucfg->blocks_per_lane, ucfg->wg_waves, ucfg->waves_per_cu, ucfg->nontemporal};
+        LaunchCfg c = to_cfg(*ucfg);
         if (!cfg_valid(c)) {
             delete b;
             return fail(GOOFY_ERR_INVALID, "invalid launch config");
@@ -284,7 +284,7 @@ int goofy_batch_get_config(const goofy_batch* b, goofy_launch_config* cfg)
 {
     clear_error();
     if (!b || !cfg) return fail(GOOFY_ERR_INVALID, "null argument");
-    *cfg = goofy_launch_config{b->cfg.blocks_per_lane, b->cfg.wg_waves, b->cfg.waves_per_cu, b->cfg.nontemporal};
+    *cfg = to_public(b->cfg);
     return GOOFY_OK;
 }
 
@@ -292,7 +292,7 @@ int goofy_batch_set_config(goofy_batch* b, const goofy_launch_config* cfg)
 {
     clear_error();
     if (!b || !cfg) return fail(GOOFY_ERR_INVALID, "null argument");
-    LaunchCfg c{cfg->blocks_per_lane, cfg->wg_waves, cfg->waves_per_cu, cfg->nontemporal};
+    LaunchCfg c = to_cfg(*cfg);
     if (!cfg_valid(c)) return fail(GOOFY_ERR_INVALID, "invalid launch config");
     const bool retile = c.blocks_per_lane != b->cfg.blocks_per_lane;
     b->cfg = c;
@@ -319,36 +319,37 @@ int goofy_batch_autotune(goofy_batch* b, void* stream, int reps, goofy_launch_co
     LaunchCfg best = b->cfg;
     float best_t = 1e30f;
     int rc = GOOFY_OK;
-    const int bpls[] = {1, 2, 4}, wgs[] = {2, 4, 8}, wpcs[] = {0, 8, 16, 32}, nts[] = {0, 1};
+    const int bpls[] = {1, 2, 4}, wgs[] = {2, 4, 8}, wpcs[] = {0, 8, 16, 32}, nts[] = {0, 1}, orders[] = {0, 1};
     for (int bpl : bpls) {
         b->cfg.blocks_per_lane = bpl;
         if ((rc = build_worklist(b)) != GOOFY_OK) break;
         for (int wg : wgs)
             for (int wpc : wpcs)
-                for (int nt : nts) {
-                    b->cfg = LaunchCfg{bpl, wg, wpc, nt};
-                    // one warm-up launch, then the median of `reps` timed ones
-                    e = launch_batch(b->cfg, b->d_descs, (uint32_t)b->descs.size(), b->ntiles, b->num_cus, s);
-                    if (e != hipSuccess) { rc = fail_hip(e, "autotune launch"); goto done; }
-                    std::vector<float> ts;
-                    for (int r = 0; r < reps; ++r) {
-                        (void)hipEventRecord(e0, s);
+                for (int nt : nts)
+                    for (int order : orders) {
+                        b->cfg = LaunchCfg{bpl, wg, wpc, nt, order};
+                        // one warm-up launch, then the median of `reps` timed ones
                         e = launch_batch(b->cfg, b->d_descs, (uint32_t)b->descs.size(), b->ntiles, b->num_cus, s);
-                        (void)hipEventRecord(e1, s);
                         if (e != hipSuccess) { rc = fail_hip(e, "autotune launch"); goto done; }
-                        e = hipEventSynchronize(e1);
-                        if (e != hipSuccess) { rc = fail_hip(e, "hipEventSynchronize"); goto done; }
-                        float ms = 0;
-                        (void)hipEventElapsedTime(&ms, e0, e1);
-                        ts.push_back(ms);
+                        std::vector<float> ts;
+                        for (int r = 0; r < reps; ++r) {
+                            (void)hipEventRecord(e0, s);
+                            e = launch_batch(b->cfg, b->d_descs, (uint32_t)b->descs.size(), b->ntiles, b->num_cus, s);
+                            (void)hipEventRecord(e1, s);
+                            if (e != hipSuccess) { rc = fail_hip(e, "autotune launch"); goto done; }
+                            e = hipEventSynchronize(e1);
+                            if (e != hipSuccess) { rc = fail_hip(e, "hipEventSynchronize"); goto done; }
+                            float ms = 0;
+                            (void)hipEventElapsedTime(&ms, e0, e1);
+                            ts.push_back(ms);
+                        }
+                        std::sort(ts.begin(), ts.end());
+                        const float med = ts[ts.size() / 2];
+                        if (med < best_t) {
+                            best_t = med;
+                            best = b->cfg;
+                        }
                     }
-                    std::sort(ts.begin(), ts.end());
-                    const float med = ts[ts.size() / 2];
-                    if (med < best_t) {
-                        best_t = med;
-                        best = b->cfg;
-                    }
-                }
     }
 done:
     (void)hipEventDestroy(e0);
@@ -365,7 +366,7 @@ done:
         tune_cache()[shape_key(b)] = best;
         save_file_cache();
     }
-    if (chosen) *chosen = goofy_launch_config{best.blocks_per_lane, best.wg_waves, best.waves_per_cu, best.nontemporal};
+    if (chosen) *chosen = to_public(best);
     if (best_ms) *best_ms = best_t;
     return GOOFY_OK;
 }

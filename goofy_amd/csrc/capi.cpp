@@ -77,14 +77,15 @@ int device_info(DeviceInfo* out)
 // Measured on MI355X (8192^2, both formats): one block per lane with
 // streaming (non-temporal) loads and stores is fastest; 1-4 wave workgroups
 // are within noise of each other, 4 blocks per lane loses ~10%.
-LaunchCfg default_plain_cfg() { return LaunchCfg{1, 2, 0, 1}; }
-LaunchCfg default_batch_cfg() { return LaunchCfg{1, 2, 0, 1}; }
+LaunchCfg default_plain_cfg() { return LaunchCfg{1, 2, 0, 1, 0}; }
+LaunchCfg default_batch_cfg() { return LaunchCfg{1, 2, 0, 1, 0}; }
 
 bool cfg_valid(const LaunchCfg& c)
 {
     const bool bpl = c.blocks_per_lane == 1 || c.blocks_per_lane == 2 || c.blocks_per_lane == 4;
     const bool wg = c.wg_waves == 1 || c.wg_waves == 2 || c.wg_waves == 4 || c.wg_waves == 8;
-    return bpl && wg && c.waves_per_cu >= 0 && c.waves_per_cu <= 256 && (c.nontemporal == 0 || c.nontemporal == 1);
+    return bpl && wg && c.waves_per_cu >= 0 && c.waves_per_cu <= 256 && (c.nontemporal == 0 || c.nontemporal == 1) &&
+           (c.tile_order == 0 || c.tile_order == 1);
 }
 
 // ---- host-memory drop-in: pinned staging, chunked pipeline ---------------------
@@ -286,7 +287,7 @@ static int compress_device(int fmt, void* d_result, const void* d_input, unsigne
     if (!magic_ok(bw, nblocks) || nblocks > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
     LaunchCfg cfg = default_plain_cfg();
     if (ucfg) {
-        cfg = LaunchCfg{ucfg->blocks_per_lane, ucfg->wg_waves, ucfg->waves_per_cu, ucfg->nontemporal};
+        cfg = to_cfg(*ucfg);
         if (!cfg_valid(cfg)) return fail(GOOFY_ERR_INVALID, "invalid launch config");
     }
     hipError_t e = launch_plain(fmt, cfg, d_input, pitch, d_result, bw, (uint32_t)nblocks, (hipStream_t)stream,
@@ -368,6 +369,54 @@ int goofy_hip_compress_device_timed(int format, void* d_result, const void* d_in
     return compress_device(format, d_result, d_input, width, height, pitch, stream, cfg, start_event, stop_event);
 }
 
+int goofy_hip_tune_device(int format, void* d_result, const void* d_input, unsigned int width, unsigned int height,
+                          size_t pitch, void* stream, int reps, goofy_launch_config* best, float* best_ms)
+{
+    // validates the arguments and warms the device with the default shape
+    int rc = compress_device(format, d_result, d_input, width, height, pitch, stream, nullptr);
+    if (rc != GOOFY_OK) return rc;
+    if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
+    if (reps < 1) reps = 1;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (hipEvent_t& x : ev) {
+        hipError_t e = hipEventCreate(&x);
+        if (e != hipSuccess) {
+            for (hipEvent_t y : ev)
+                if (y) (void)hipEventDestroy(y);
+            return fail_hip(e, "hipEventCreate");
+        }
+    }
+    LaunchCfg top = default_plain_cfg();
+    float top_ms = 1e30f;
+    const int bpls[] = {1, 2, 4}, wgs[] = {1, 2, 4, 8}, orders[] = {0, 1};
+    for (int bpl : bpls)
+        for (int wg : wgs)
+            for (int order : orders) {
+                const goofy_launch_config c = to_public(LaunchCfg{bpl, wg, 0, 1, order});
+                std::vector<float> ts;
+                for (int r = 0; r <= reps && rc == GOOFY_OK; ++r) {  // r == 0: warm-up
+                    rc = compress_device(format, d_result, d_input, width, height, pitch, stream, &c, ev[0], ev[1]);
+                    hipError_t e = rc == GOOFY_OK ? hipEventSynchronize(ev[1]) : hipSuccess;
+                    if (e != hipSuccess) rc = fail_hip(e, "hipEventSynchronize");
+                    float ms = 0;
+                    if (rc == GOOFY_OK && r > 0 && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess)
+                        ts.push_back(ms);
+                }
+                if (rc != GOOFY_OK) break;
+                if (ts.empty()) continue;
+                std::sort(ts.begin(), ts.end());
+                if (ts[ts.size() / 2] < top_ms) {
+                    top_ms = ts[ts.size() / 2];
+                    top = to_cfg(c);
+                }
+            }
+    for (hipEvent_t y : ev) (void)hipEventDestroy(y);
+    if (rc != GOOFY_OK) return rc;
+    if (best) *best = to_public(top);
+    if (best_ms) *best_ms = top_ms;
+    return GOOFY_OK;
+}
+
 static int check_image_device(int fmt, const void* d_img, const void* d_blocks, unsigned width, unsigned height,
                               size_t pitch)
 {
@@ -402,31 +451,29 @@ static double psnr_of(double mse, double peak)
     return 10.0 * std::log10((peak * peak) / mse);
 }
 
-// The quality sums live in a 64-byte device buffer: 4 x uint64 byte sums,
-// then the double luma sum.  Reads them back on `stream` and fills `out` with
-// getMsePsnr's figures (Src/main.cpp:403-469).
+// Runs `launch(scratch)` -- a quality launcher (QualitySums at offset 0 of
+// its scratch buffer when done) -- reads the sums back on `stream` and fills
+// `out` with getMsePsnr's figures (Src/main.cpp:403-469).
 extern "C++" template <typename Launch>
-static int run_quality(uint64_t pixels, hipStream_t stream, goofy_quality* out, Launch&& launch)
+static int run_quality(uint64_t pixels, uint64_t units, hipStream_t stream, goofy_quality* out, Launch&& launch)
 {
     void* d = nullptr;
-    hipError_t e = hipMalloc(&d, 64);
+    hipError_t e = hipMalloc(&d, quality_scratch_bytes(units));
     if (e != hipSuccess) return fail_hip(e, "hipMalloc(quality sums)");
-    unsigned long long* sums = static_cast<unsigned long long*>(d);
-    double* sum_y = reinterpret_cast<double*>(sums + 4);
     struct Free {
         void* p;
         ~Free() { (void)hipFree(p); }
     } guard{d};
-    e = launch(sums, sum_y);
+    e = launch(d);
     if (e != hipSuccess) return fail_hip(e, "quality kernel launch");
-    unsigned long long h[5];
-    e = hipMemcpyAsync(h, d, 40, hipMemcpyDeviceToHost, stream);
+    QualitySums h;
+    e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return fail_hip(e, "quality readback");
     goofy_quality q{};
     q.pixels = pixels;
-    for (int i = 0; i < 4; ++i) q.sse_byte[i] = h[i];
-    std::memcpy(&q.sse_luma, &h[4], sizeof(double));
+    for (int i = 0; i < 4; ++i) q.sse_byte[i] = h.e[i];
+    q.sse_luma = h.y;
     const double n = (double)q.pixels;
     // labels as in the reference harness: "r" = byte 2, "b" = byte 0
     q.mse_r = (double)q.sse_byte[2] / n;
@@ -455,8 +502,9 @@ int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_sou
     int rc = check_image_device(format, d_source, d_blocks, width, height, pitch);
     if (rc != GOOFY_OK) return rc;
     if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
-    return run_quality((uint64_t)width * height, (hipStream_t)stream, out, [&](unsigned long long* sums, double* sy) {
-        return launch_sqerr(format, d_blocks, d_source, pitch, width, height, sums, sy, (hipStream_t)stream);
+    const uint64_t nblocks = (uint64_t)(width / 4) * (height / 4);
+    return run_quality((uint64_t)width * height, nblocks, (hipStream_t)stream, out, [&](void* scratch) {
+        return launch_sqerr(format, d_blocks, d_source, pitch, width, height, scratch, (hipStream_t)stream);
     });
 }
 
@@ -488,8 +536,9 @@ int goofy_hip_quality_images_device(const void* d_a, size_t pitch_a, const void*
     int rc = check_rgba_pair(d_a, pitch_a, d_b, pitch_b, width, height, 1);
     if (rc != GOOFY_OK) return rc;
     if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
-    return run_quality((uint64_t)width * height, (hipStream_t)stream, out, [&](unsigned long long* sums, double* sy) {
-        return launch_sqerr_images(d_a, pitch_a, d_b, pitch_b, width, height, sums, sy, (hipStream_t)stream);
+    const uint64_t nquads = (uint64_t)(width / 4) * height;
+    return run_quality((uint64_t)width * height, nquads, (hipStream_t)stream, out, [&](void* scratch) {
+        return launch_sqerr_images(d_a, pitch_a, d_b, pitch_b, width, height, scratch, (hipStream_t)stream);
     });
 }
 

@@ -170,44 +170,56 @@ __device__ __forceinline__ void accum_pixel(uint32_t a, uint32_t d, uint32_t (&e
     ey += dy * dy;
 }
 
-// Wave reduction, then one atomic per wave and sum.  Every lane of the wave
-// must reach this (no early return before it).
-__device__ __forceinline__ void flush_sums(const uint32_t (&e)[4], double ey, unsigned long long* sums,
-                                           double* sums_y)
+// Sums of one workgroup (256 lanes = 4 waves): wave reduction by lane
+// exchange, then the four wave sums in a fixed order, stored by lane 0 with a
+// plain vector store.  Every lane of the workgroup must reach this.  Fixed
+// orders everywhere make the double luma sum the same on every run.
+__device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], double ey, QualitySums* out)
 {
-    unsigned long long w[4] = {e[0], e[1], e[2], e[3]};
+    __shared__ unsigned long long s_e[4][4];
+    __shared__ double s_y[4];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-        for (int ch = 0; ch < 4; ++ch) w[ch] += __shfl_xor(w[ch], off, 64);
+        for (int ch = 0; ch < 4; ++ch) e[ch] += __shfl_xor(e[ch], off, 64);
         ey += __shfl_xor(ey, off, 64);
     }
+    const uint32_t wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 0) {
 #pragma unroll
-        for (int ch = 0; ch < 4; ++ch) atomicAdd(&sums[ch], w[ch]);
-        atomicAdd(sums_y, ey);
+        for (int ch = 0; ch < 4; ++ch) s_e[wave][ch] = e[ch];
+        s_y[wave] = ey;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        QualitySums r;
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) r.e[ch] = s_e[0][ch] + s_e[1][ch] + s_e[2][ch] + s_e[3][ch];
+        r.y = ((s_y[0] + s_y[1]) + s_y[2]) + s_y[3];
+        *out = r;
     }
 }
 
 // Round-trip error: decode each block in registers and compare it with the
-// source pixels.  sums[0..3] = squared error of bytes 0..3 of the pixels
-// (uint64), sums_y = sum of squared luma differences with the harness's
-// weights (0.21, 0.72, 0.07 on bytes 2, 1, 0 -- getMsePsnr reads a pixel as
-// a little-endian word and takes bits 16..23 as "red").
+// source pixels.  Per workgroup: e[0..3] = squared error of bytes 0..3 of the
+// pixels, y = sum of squared luma differences with the harness's weights
+// (0.21, 0.72, 0.07 on bytes 2, 1, 0 -- getMsePsnr reads a pixel as a
+// little-endian word and takes bits 16..23 as "red").  Grid-stride over the
+// blocks, one partial per workgroup; k_sums_final adds the partials.
 template <int FMT>
 __global__ __launch_bounds__(256) void k_sqerr(const uint2* __restrict__ blocks, const uint8_t* __restrict__ src,
                                                uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
-                                               unsigned long long* __restrict__ sums, double* __restrict__ sums_y)
+                                               QualitySums* __restrict__ part)
 {
 #pragma clang fp contract(off)
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t e[4] = {0, 0, 0, 0};
+    unsigned long long acc[4] = {0, 0, 0, 0};
     double ey = 0.0;
-    if (i < nblocks) {
-        const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < nblocks; i += (uint64_t)gridDim.x * 256u) {
+        const uint32_t by = (uint32_t)((i * magic) >> 40), bx = (uint32_t)i - by * bw;
         const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
         uint32_t px[16];
         decode_block<FMT>(load_block(blocks + i), px);
+        uint32_t e[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
@@ -215,34 +227,52 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint2* __restrict__ blocks,
 #pragma unroll
             for (int c = 0; c < 4; ++c) accum_pixel(s[c], px[4 * r + c], e, ey);
         }
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) acc[ch] += e[ch];
     }
-    flush_sums(e, ey, sums, sums_y);
+    block_sums_store(acc, ey, part + blockIdx.x);
 }
 
 // Squared errors of two whole RGBA8 images (getMsePsnr over a source and any
-// other image of its size).  One lane per 4 pixels of one row.
+// other image of its size).  One lane-step per 4 pixels of one row.
 __global__ __launch_bounds__(256) void k_sqerr_images(const uint8_t* __restrict__ a, uint64_t pitch_a,
                                                       const uint8_t* __restrict__ b, uint64_t pitch_b, uint32_t qw,
                                                       uint32_t nquads, uint64_t magic,
-                                                      unsigned long long* __restrict__ sums,
-                                                      double* __restrict__ sums_y)
+                                                      QualitySums* __restrict__ part)
 {
 #pragma clang fp contract(off)
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t e[4] = {0, 0, 0, 0};
+    unsigned long long acc[4] = {0, 0, 0, 0};
     double ey = 0.0;
-    if (i < nquads) {
-        const uint32_t y = (uint32_t)(((uint64_t)i * magic) >> 40), x = i - y * qw;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < nquads; i += (uint64_t)gridDim.x * 256u) {
+        const uint32_t y = (uint32_t)((i * magic) >> 40), x = (uint32_t)i - y * qw;
         const u32x4 va =
             __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a + (uint64_t)y * pitch_a + (uint64_t)x * 16u));
         const u32x4 vb =
             __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(b + (uint64_t)y * pitch_b + (uint64_t)x * 16u));
+        uint32_t e[4] = {0, 0, 0, 0};
         accum_pixel(va.x, vb.x, e, ey);
         accum_pixel(va.y, vb.y, e, ey);
         accum_pixel(va.z, vb.z, e, ey);
         accum_pixel(va.w, vb.w, e, ey);
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) acc[ch] += e[ch];
     }
-    flush_sums(e, ey, sums, sums_y);
+    block_sums_store(acc, ey, part + blockIdx.x);
+}
+
+// One workgroup adds the g partials in a fixed order into *out.
+__global__ __launch_bounds__(256) void k_sums_final(const QualitySums* __restrict__ part, uint32_t g,
+                                                    QualitySums* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    unsigned long long acc[4] = {0, 0, 0, 0};
+    double ey = 0.0;
+    for (uint32_t k = threadIdx.x; k < g; k += 256u) {
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) acc[ch] += part[k].e[ch];
+        ey += part[k].y;
+    }
+    block_sums_store(acc, ey, out);
 }
 
 // One 2x2 cell of the quality floor (Src/main.cpp:507-553).  p and q are the
@@ -285,12 +315,6 @@ __global__ __launch_bounds__(256) void k_rgb565_baseline(const uint8_t* __restri
     __builtin_nontemporal_store(o, reinterpret_cast<u32x4*>(d0 + dst_pitch));
 }
 
-__global__ void k_zero_sums(unsigned long long* sums, double* sums_y)
-{
-    if (threadIdx.x < 4) sums[threadIdx.x] = 0ull;
-    if (threadIdx.x == 4) *sums_y = 0.0;
-}
-
 }  // namespace
 
 hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width, uint32_t height, size_t pitch,
@@ -310,30 +334,33 @@ hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width,
 }
 
 hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
-                        unsigned long long* d_sums, double* d_sum_y, hipStream_t stream)
+                        void* scratch, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_zero_sums, dim3(1), dim3(64), 0, stream, d_sums, d_sum_y);
     const uint32_t bw = width / 4, nblocks = bw * (height / 4);
-    if (nblocks == 0) return hipGetLastError();
-    const dim3 grid((nblocks + 255u) / 256u), block(256);
+    const uint32_t g = quality_grid(nblocks);
+    QualitySums* out = static_cast<QualitySums*>(scratch);
+    QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(scratch) + kQualityPartialsOffset);
     const uint64_t magic = div_magic(bw);
     if (fmt == 0)
-        hipLaunchKernelGGL(k_sqerr<0>, grid, block, 0, stream, (const uint2*)blocks, (const uint8_t*)src,
-                           (uint64_t)pitch, bw, nblocks, magic, d_sums, d_sum_y);
+        hipLaunchKernelGGL(k_sqerr<0>, dim3(g), dim3(256), 0, stream, (const uint2*)blocks, (const uint8_t*)src,
+                           (uint64_t)pitch, bw, nblocks, magic, part);
     else
-        hipLaunchKernelGGL(k_sqerr<1>, grid, block, 0, stream, (const uint2*)blocks, (const uint8_t*)src,
-                           (uint64_t)pitch, bw, nblocks, magic, d_sums, d_sum_y);
+        hipLaunchKernelGGL(k_sqerr<1>, dim3(g), dim3(256), 0, stream, (const uint2*)blocks, (const uint8_t*)src,
+                           (uint64_t)pitch, bw, nblocks, magic, part);
+    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out);
     return hipGetLastError();
 }
 
 hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
-                               uint32_t height, unsigned long long* d_sums, double* d_sum_y, hipStream_t stream)
+                               uint32_t height, void* scratch, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_zero_sums, dim3(1), dim3(64), 0, stream, d_sums, d_sum_y);
     const uint32_t qw = width / 4, n = qw * height;
-    if (n == 0) return hipGetLastError();
-    hipLaunchKernelGGL(k_sqerr_images, dim3((n + 255u) / 256u), dim3(256), 0, stream, (const uint8_t*)a,
-                       (uint64_t)pitch_a, (const uint8_t*)b, (uint64_t)pitch_b, qw, n, div_magic(qw), d_sums, d_sum_y);
+    const uint32_t g = quality_grid(n);
+    QualitySums* out = static_cast<QualitySums*>(scratch);
+    QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(scratch) + kQualityPartialsOffset);
+    hipLaunchKernelGGL(k_sqerr_images, dim3(g), dim3(256), 0, stream, (const uint8_t*)a, (uint64_t)pitch_a,
+                       (const uint8_t*)b, (uint64_t)pitch_b, qw, n, div_magic(qw), part);
+    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out);
     return hipGetLastError();
 }
 

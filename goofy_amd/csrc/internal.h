@@ -33,7 +33,22 @@ struct LaunchCfg {
     int wg_waves;         // waves per workgroup: 1, 2, 4 or 8
     int waves_per_cu;     // persistent grid size (0 = one wave per tile, no persistence)
     int nontemporal;      // 1 = streaming (non-temporal) loads and stores
+    int tile_order;       // 0 = linear, 1 = XCD-contiguous (kXcdCount groups of workgroups)
 };
+
+// The dispatcher hands workgroup b to XCD b % 8.  With tile_order 1, kernels
+// renumber workgroups so that each XCD's workgroups take one contiguous
+// eighth of the tiles, in order.
+constexpr uint32_t kXcdCount = 8;
+
+inline LaunchCfg to_cfg(const goofy_launch_config& u)
+{
+    return LaunchCfg{u.blocks_per_lane, u.wg_waves, u.waves_per_cu, u.nontemporal, u.tile_order};
+}
+inline goofy_launch_config to_public(const LaunchCfg& c)
+{
+    return goofy_launch_config{c.blocks_per_lane, c.wg_waves, c.waves_per_cu, c.nontemporal, c.tile_order};
+}
 
 // A small work-list passed by value in the kernel arguments (no copy to the
 // device before the launch): up to kInlineTextures descriptors and their
@@ -69,10 +84,29 @@ hipError_t launch_synth(const void* params, void* dst, uint32_t width, uint32_t 
                         hipStream_t stream);
 hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width, uint32_t height, size_t pitch,
                          hipStream_t stream);
+// Quality sums: squared errors of bytes 0..3 and the double luma sum.  The
+// quality launchers take a device scratch buffer of quality_scratch_bytes():
+// the final sums at offset 0, one partial per workgroup from
+// kQualityPartialsOffset on.
+struct QualitySums {
+    unsigned long long e[4];
+    double y;
+};
+constexpr size_t kQualityPartialsOffset = 64;
+constexpr uint32_t kQualityMaxGroups = 2048;
+inline uint32_t quality_grid(uint64_t units)
+{
+    const uint64_t g = (units + 255u) / 256u;
+    return (uint32_t)(g < 1 ? 1 : (g > kQualityMaxGroups ? kQualityMaxGroups : g));
+}
+inline size_t quality_scratch_bytes(uint64_t units)
+{
+    return kQualityPartialsOffset + (size_t)quality_grid(units) * sizeof(QualitySums);
+}
 hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
-                        unsigned long long* d_sums, double* d_sum_y, hipStream_t stream);
+                        void* scratch, hipStream_t stream);
 hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
-                               uint32_t height, unsigned long long* d_sums, double* d_sum_y, hipStream_t stream);
+                               uint32_t height, void* scratch, hipStream_t stream);
 hipError_t launch_rgb565_baseline(const void* src, size_t src_pitch, void* dst, size_t dst_pitch, uint32_t width,
                                   uint32_t height, hipStream_t stream, hipEvent_t start = nullptr,
                                   hipEvent_t stop = nullptr);

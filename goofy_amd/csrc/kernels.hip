@@ -77,13 +77,30 @@ __device__ __forceinline__ void encode_tile(const uint8_t* __restrict__ src, uin
     }
 }
 
+// Index of this wave in the launch.  With order 1 the workgroups are
+// renumbered so that those the dispatcher places on XCD x (b % 8 == x) take
+// the x-th contiguous share of the launch, in order: each XCD then streams
+// through one contiguous region instead of every eighth workgroup's tiles.
+// A bijection on [0, gridDim.x): r = n % 8 XCDs hold q + 1 workgroups, the
+// rest q.  All operands are wave-uniform (scalar unit).
+__device__ __forceinline__ uint32_t wave_index(uint32_t order)
+{
+    uint32_t b = blockIdx.x;
+    if (order) {
+        const uint32_t n = gridDim.x, q = n / kXcdCount, r = n % kXcdCount;
+        const uint32_t x = b % kXcdCount, k = b / kXcdCount;
+        b = x < r ? x * (q + 1u) + k : r * (q + 1u) + (x - r) * q + k;
+    }
+    return __builtin_amdgcn_readfirstlane(b * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
+
 // One image; wave w encodes tile w.
 template <int FMT, int BPL, bool NT>
 __global__ __launch_bounds__(512) void k_encode_plain(const uint8_t* __restrict__ src, uint64_t pitch,
                                                       uint8_t* __restrict__ dst, uint32_t bw, uint32_t nblocks,
-                                                      uint64_t magic)
+                                                      uint64_t magic, uint32_t order)
 {
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const uint32_t wave = wave_index(order);
     encode_tile<FMT, BPL, NT>(src, pitch, dst, bw, magic, nblocks, wave * 64u * BPL, threadIdx.x & 63u);
 }
 
@@ -97,9 +114,9 @@ __global__ __launch_bounds__(512) void k_encode_plain(const uint8_t* __restrict_
 // format branch never diverges.
 template <int BPL, bool NT, class Descs, class Begins>
 __device__ __forceinline__ void batch_body(const Descs& descs, const Begins& tile_begin, uint32_t ntex,
-                                           uint32_t ntiles, uint32_t tpw)
+                                           uint32_t ntiles, uint32_t tpw, uint32_t order)
 {
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const uint32_t wave = wave_index(order);
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t t = wave * tpw;
     const uint32_t t_end = min(t + tpw, ntiles);
@@ -129,17 +146,17 @@ __device__ __forceinline__ void batch_body(const Descs& descs, const Begins& til
 template <int BPL, bool NT>
 __global__ __launch_bounds__(512) void k_encode_batch(const TexDesc* __restrict__ descs,
                                                       const uint32_t* __restrict__ tile_begin, uint32_t ntex,
-                                                      uint32_t ntiles, uint32_t tpw)
+                                                      uint32_t ntiles, uint32_t tpw, uint32_t order)
 {
-    batch_body<BPL, NT>(descs, tile_begin, ntex, ntiles, tpw);
+    batch_body<BPL, NT>(descs, tile_begin, ntex, ntiles, tpw, order);
 }
 
 // The same with the work-list in the kernel arguments (small groups).
 template <int BPL, bool NT>
 __global__ __launch_bounds__(512) void k_encode_batch_inline(const InlineWorklist wl, uint32_t ntex, uint32_t ntiles,
-                                                             uint32_t tpw)
+                                                             uint32_t tpw, uint32_t order)
 {
-    batch_body<BPL, NT>(wl.d, wl.tile_begin, ntex, ntiles, tpw);
+    batch_body<BPL, NT>(wl.d, wl.tile_begin, ntex, ntiles, tpw, order);
 }
 
 __global__ __launch_bounds__(256) void k_synth(goofy_synth::Params P, uint8_t* __restrict__ dst, uint32_t width,
@@ -158,14 +175,15 @@ __global__ __launch_bounds__(256) void k_synth(goofy_synth::Params P, uint8_t* _
 // the kernel's execution, without marker packets around it.
 template <int FMT, int BPL, bool NT>
 static hipError_t plain_go(dim3 grid, dim3 block, const void* src, size_t pitch, void* dst, uint32_t bw,
-                           uint32_t nblocks, uint64_t magic, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+                           uint32_t nblocks, uint64_t magic, uint32_t order, hipStream_t s, hipEvent_t e0,
+                           hipEvent_t e1)
 {
     if (e0 || e1)
         hipExtLaunchKernelGGL((k_encode_plain<FMT, BPL, NT>), grid, block, 0, s, e0, e1, 0, (const uint8_t*)src,
-                              (uint64_t)pitch, (uint8_t*)dst, bw, nblocks, magic);
+                              (uint64_t)pitch, (uint8_t*)dst, bw, nblocks, magic, order);
     else
         hipLaunchKernelGGL((k_encode_plain<FMT, BPL, NT>), grid, block, 0, s, (const uint8_t*)src, (uint64_t)pitch,
-                           (uint8_t*)dst, bw, nblocks, magic);
+                           (uint8_t*)dst, bw, nblocks, magic, order);
     return hipGetLastError();
 }
 
@@ -175,16 +193,17 @@ static hipError_t plain_bpl(const LaunchCfg& c, dim3 grid, dim3 block, const voi
                             hipEvent_t e1)
 {
     const bool nt = c.nontemporal != 0;
+    const uint32_t o = (uint32_t)c.tile_order;
     switch (c.blocks_per_lane) {
     case 1:
-        return nt ? plain_go<FMT, 1, true>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1)
-                  : plain_go<FMT, 1, false>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1);
+        return nt ? plain_go<FMT, 1, true>(grid, block, src, pitch, dst, bw, nblocks, magic, o, s, e0, e1)
+                  : plain_go<FMT, 1, false>(grid, block, src, pitch, dst, bw, nblocks, magic, o, s, e0, e1);
     case 2:
-        return nt ? plain_go<FMT, 2, true>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1)
-                  : plain_go<FMT, 2, false>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1);
+        return nt ? plain_go<FMT, 2, true>(grid, block, src, pitch, dst, bw, nblocks, magic, o, s, e0, e1)
+                  : plain_go<FMT, 2, false>(grid, block, src, pitch, dst, bw, nblocks, magic, o, s, e0, e1);
     default:
-        return nt ? plain_go<FMT, 4, true>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1)
-                  : plain_go<FMT, 4, false>(grid, block, src, pitch, dst, bw, nblocks, magic, s, e0, e1);
+        return nt ? plain_go<FMT, 4, true>(grid, block, src, pitch, dst, bw, nblocks, magic, o, s, e0, e1)
+                  : plain_go<FMT, 4, false>(grid, block, src, pitch, dst, bw, nblocks, magic, o, s, e0, e1);
     }
 }
 
@@ -204,10 +223,10 @@ hipError_t launch_plain(int fmt, const LaunchCfg& c, const void* src, size_t pit
 
 template <int BPL>
 static hipError_t batch_nt(bool nt, dim3 grid, dim3 block, const TexDesc* d, const uint32_t* tb, uint32_t ntex,
-                           uint32_t ntiles, uint32_t tpw, hipStream_t s)
+                           uint32_t ntiles, uint32_t tpw, uint32_t order, hipStream_t s)
 {
-    if (nt) hipLaunchKernelGGL((k_encode_batch<BPL, true>), grid, block, 0, s, d, tb, ntex, ntiles, tpw);
-    else hipLaunchKernelGGL((k_encode_batch<BPL, false>), grid, block, 0, s, d, tb, ntex, ntiles, tpw);
+    if (nt) hipLaunchKernelGGL((k_encode_batch<BPL, true>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order);
+    else hipLaunchKernelGGL((k_encode_batch<BPL, false>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order);
     return hipGetLastError();
 }
 
@@ -223,18 +242,18 @@ hipError_t launch_batch(const LaunchCfg& c, const TexDesc* d_descs, uint32_t nte
     const uint32_t wg = (uint32_t)c.wg_waves;
     const dim3 grid((uint32_t)((waves + wg - 1) / wg)), block(64u * wg);
     switch (c.blocks_per_lane) {
-    case 1: return batch_nt<1>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, stream);
-    case 2: return batch_nt<2>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, stream);
-    default: return batch_nt<4>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, stream);
+    case 1: return batch_nt<1>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, c.tile_order, stream);
+    case 2: return batch_nt<2>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, c.tile_order, stream);
+    default: return batch_nt<4>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, c.tile_order, stream);
     }
 }
 
 template <int BPL>
 static hipError_t batch_inline_nt(bool nt, dim3 grid, dim3 block, const InlineWorklist& wl, uint32_t ntex,
-                                  uint32_t ntiles, uint32_t tpw, hipStream_t s)
+                                  uint32_t ntiles, uint32_t tpw, uint32_t order, hipStream_t s)
 {
-    if (nt) hipLaunchKernelGGL((k_encode_batch_inline<BPL, true>), grid, block, 0, s, wl, ntex, ntiles, tpw);
-    else hipLaunchKernelGGL((k_encode_batch_inline<BPL, false>), grid, block, 0, s, wl, ntex, ntiles, tpw);
+    if (nt) hipLaunchKernelGGL((k_encode_batch_inline<BPL, true>), grid, block, 0, s, wl, ntex, ntiles, tpw, order);
+    else hipLaunchKernelGGL((k_encode_batch_inline<BPL, false>), grid, block, 0, s, wl, ntex, ntiles, tpw, order);
     return hipGetLastError();
 }
 
@@ -250,9 +269,9 @@ hipError_t launch_batch_inline(const LaunchCfg& c, const InlineWorklist& wl, uin
     const uint32_t wg = (uint32_t)c.wg_waves;
     const dim3 grid((uint32_t)((waves + wg - 1) / wg)), block(64u * wg);
     switch (c.blocks_per_lane) {
-    case 1: return batch_inline_nt<1>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, stream);
-    case 2: return batch_inline_nt<2>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, stream);
-    default: return batch_inline_nt<4>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, stream);
+    case 1: return batch_inline_nt<1>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, c.tile_order, stream);
+    case 2: return batch_inline_nt<2>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, c.tile_order, stream);
+    default: return batch_inline_nt<4>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, c.tile_order, stream);
     }
 }
 

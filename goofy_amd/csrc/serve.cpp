@@ -354,8 +354,9 @@ int goofy_server_create(const goofy_server_config* ucfg, goofy_server** out)
     if (c.max_pending <= 0) c.max_pending = 65536;
     if (c.streams > 64 || c.max_batch_textures > 65536) return fail(GOOFY_ERR_INVALID, "streams <= 64, batch <= 65536");
     LaunchCfg lc = default_batch_cfg();
-    if (c.launch.blocks_per_lane || c.launch.wg_waves || c.launch.waves_per_cu || c.launch.nontemporal) {
-        lc = LaunchCfg{c.launch.blocks_per_lane, c.launch.wg_waves, c.launch.waves_per_cu, c.launch.nontemporal};
+    if (c.launch.blocks_per_lane || c.launch.wg_waves || c.launch.waves_per_cu || c.launch.nontemporal ||
+        c.launch.tile_order) {
+        lc = to_cfg(c.launch);
         if (!cfg_valid(lc)) return fail(GOOFY_ERR_INVALID, "invalid launch config");
     }
     DeviceInfo di;

@@ -75,6 +75,10 @@ typedef struct goofy_launch_config {
     int wg_waves;        /* waves per workgroup: 1, 2, 4 or 8 */
     int waves_per_cu;    /* persistent grid: waves per CU (0 = one wave per tile) */
     int nontemporal;     /* 1 = streaming loads/stores */
+    int tile_order;      /* 0 = linear wave->tile order; 1 = XCD-contiguous: the
+                            workgroups the dispatcher places on each of the 8
+                            XCDs (workgroup b on XCD b % 8) take one contiguous
+                            eighth of the tiles */
 } goofy_launch_config;
 
 /* As goofy_hip_compress_device with an explicit launch shape (NULL = default). */
@@ -88,6 +92,17 @@ GOOFY_HIP_API int goofy_hip_compress_device_cfg(int format, void* d_result, cons
 GOOFY_HIP_API int goofy_hip_compress_device_timed(int format, void* d_result, const void* d_input,
                                                   unsigned int width, unsigned int height, size_t pitch, void* stream,
                                                   const goofy_launch_config* cfg, void* start_event, void* stop_event);
+
+/* Launch-shape autotuner for one image shape.  Times the encode kernel under
+ * each candidate shape (1, 2 or 4 blocks per lane; workgroups of 1, 2, 4 or 8
+ * waves; both tile orders; streaming loads and stores) on the caller's
+ * buffers -- d_result ends up holding the image's encoding -- and returns the
+ * shape with the lowest median kernel time over `reps` timed launches, and
+ * that time.  Pass the shape to goofy_hip_compress_device_cfg for later
+ * images of the same size and pitch.  Synchronises `stream`. */
+GOOFY_HIP_API int goofy_hip_tune_device(int format, void* d_result, const void* d_input, unsigned int width,
+                                        unsigned int height, size_t pitch, void* stream, int reps,
+                                        goofy_launch_config* best, float* best_ms);
 
 /* ---- decode and round-trip quality ------------------------------------------
  * Decoders with the semantics of the reference's quality harness

@@ -81,6 +81,11 @@ def test_device_api_validation_without_device():
             goofy_amd.compress_device(*args)
         assert e.value.status == goofy_amd.ERR_INVALID, args
     with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.compress_device(goofy_amd.DXT1, 8, 16, 4, 4, 16, config={"blocks_per_lane": 1, "wg_waves": 2,
+                                                                            "waves_per_cu": 0, "nontemporal": 1,
+                                                                            "tile_order": 2})
+    assert e.value.status == goofy_amd.ERR_INVALID
+    with pytest.raises(goofy_amd.GoofyError) as e:
         goofy_amd.compress_device(goofy_amd.DXT1, 8, 16, 4, 4, 16, config={"blocks_per_lane": 3, "wg_waves": 4,
                                                                             "waves_per_cu": 0, "nontemporal": 0})
     assert e.value.status == goofy_amd.ERR_INVALID

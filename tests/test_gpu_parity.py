@@ -117,6 +117,11 @@ def test_device_large_images_golden(gpu, digests, name):
         say(f"device {name} {key}")
         out = encode_dev(gpu, fmt, src, im.width, im.height, im.pitch)
         assert sha(out) == d[f"{key}_sha256"], key
+        # the XCD-contiguous tile order on the same image
+        cfg = dict(blocks_per_lane=4, wg_waves=8, waves_per_cu=0, nontemporal=1, tile_order=1)
+        say(f"device {name} {key} {cfg}")
+        out = encode_dev(gpu, fmt, src, im.width, im.height, im.pitch, cfg)
+        assert sha(out) == d[f"{key}_sha256"], (key, cfg)
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -141,14 +146,31 @@ def test_device_shapes_vs_oracle(gpu, oracle, seed):
 @pytest.mark.parametrize("bpl", [1, 2, 4])
 @pytest.mark.parametrize("wg", [1, 2, 4, 8])
 @pytest.mark.parametrize("nt", [0, 1])
-def test_launch_configs_vs_oracle(gpu, oracle, bpl, wg, nt):
+@pytest.mark.parametrize("order", [0, 1])
+def test_launch_configs_vs_oracle(gpu, oracle, bpl, wg, nt, order):
+    # 257 x 65 blocks: the tile count is not a multiple of any workgroup size
+    # or of the 8 XCDs, so every remapped launch has a partial last share
     w, h, pitch = 1028, 260, 1028 * 4 + 48
     img = gpu.synth_host(0, 1234, 0, 25, w, h, pitch, fill=0x77)
     src = upload(img)
-    cfg = dict(blocks_per_lane=bpl, wg_waves=wg, waves_per_cu=0, nontemporal=nt)
+    cfg = dict(blocks_per_lane=bpl, wg_waves=wg, waves_per_cu=0, nontemporal=nt, tile_order=order)
     for fmt in (W.DXT1, W.ETC1):
         say(f"config {cfg} fmt={fmt}")
         assert np.array_equal(encode_dev(gpu, fmt, src, w, h, pitch, cfg), oracle.compress_blocks(fmt, img, w, h, pitch))
+
+
+def test_tune_device_picks_a_valid_shape(gpu, oracle):
+    w, h, pitch = 2048, 1024, 2048 * 4 + 256
+    img = gpu.synth_host(0, 99, 0, 12, w, h, pitch, fill=0)
+    src = upload(img)
+    out = dev_buffer((w // 4) * (h // 4) * 8, fill=0xCD)
+    cfg, ms = gpu.tune_device(W.ETC1, out, src, w, h, pitch, reps=2)
+    say(f"tuned {cfg} {ms:.4f} ms")
+    want = oracle.compress_blocks(W.ETC1, img, w, h, pitch)
+    assert ms > 0 and cfg["blocks_per_lane"] in (1, 2, 4) and cfg["tile_order"] in (0, 1)
+    assert np.array_equal(to_host(out), want)  # the tuner leaves a valid encoding behind
+    out.fill(0xCD)
+    assert np.array_equal(encode_dev(gpu, W.ETC1, src, w, h, pitch, cfg), want)
 
 
 # ---- batched --------------------------------------------------------------------------------
@@ -212,7 +234,9 @@ def test_batch_vs_oracle_and_configs(gpu, oracle):
     check("per-texture")
     for cfg in (dict(blocks_per_lane=1, wg_waves=2, waves_per_cu=0, nontemporal=0),
                 dict(blocks_per_lane=4, wg_waves=8, waves_per_cu=1, nontemporal=1),
-                dict(blocks_per_lane=2, wg_waves=1, waves_per_cu=3, nontemporal=0)):
+                dict(blocks_per_lane=2, wg_waves=1, waves_per_cu=3, nontemporal=0),
+                dict(blocks_per_lane=1, wg_waves=2, waves_per_cu=0, nontemporal=1, tile_order=1),
+                dict(blocks_per_lane=2, wg_waves=4, waves_per_cu=5, nontemporal=1, tile_order=1)):
         clear()
         b.config = cfg
         say(f"batch config {cfg}")
