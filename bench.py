@@ -468,7 +468,7 @@ def main() -> int:
 
     cfg = None
     if args.bpl or args.wg or args.nt >= 0:
-        cfg = dict(blocks_per_lane=args.bpl or 1, wg_waves=args.wg or 2, waves_per_cu=0,
+        cfg = dict(blocks_per_lane=args.bpl or 1, wg_waves=args.wg or 8, waves_per_cu=0,
                    nontemporal=1 if args.nt < 0 else args.nt)
 
     im, elapsed, kern_ms, kern_med, parity = bench_plain(g, hip, W, args.steps, args.warmup, args.rotate, cfg)

@@ -236,7 +236,7 @@ int goofy_batch_create(const goofy_texture_desc* textures, size_t count, const g
         std::lock_guard<std::mutex> lock(g_tune_mu);
         load_file_cache();
         auto it = tune_cache().find(shape_key(b));
-        b->cfg = it != tune_cache().end() ? it->second : default_batch_cfg();
+        b->cfg = it != tune_cache().end() ? it->second : auto_cfg(b->blocks, b->num_cus);
     }
     if (ucfg) {
         LaunchCfg c = to_cfg(*ucfg);
@@ -271,9 +271,9 @@ int goofy_batch_launch_per_texture(goofy_batch* b, void* stream)
 {
     clear_error();
     if (!b) return fail(GOOFY_ERR_INVALID, "null batch");
-    const LaunchCfg c = default_plain_cfg();
     for (const goofy_texture_desc& t : b->textures) {
         const uint32_t bw = t.width / 4, bh = t.height / 4;
+        const LaunchCfg c = auto_cfg((uint64_t)bw * bh, b->num_cus);
         hipError_t e = launch_plain(t.format, c, t.src, t.pitch, t.dst, bw, bw * bh, (hipStream_t)stream);
         if (e != hipSuccess) return fail_hip(e, "per-texture launch");
     }

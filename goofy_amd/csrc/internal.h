@@ -126,8 +126,9 @@ struct DeviceInfo {
 int device_info(DeviceInfo* out);  // current device; returns GOOFY_OK or an error code
 
 inline int tile_blocks(const LaunchCfg& c) { return 64 * c.blocks_per_lane; }
-LaunchCfg default_plain_cfg();
-LaunchCfg default_batch_cfg();
+// Default launch shape for a launch of `blocks` 4x4 blocks on a device with
+// num_cus compute units (used wherever the caller gives no shape).
+LaunchCfg auto_cfg(uint64_t blocks, int num_cus);
 bool cfg_valid(const LaunchCfg& c);
 
 }  // namespace goofy_impl

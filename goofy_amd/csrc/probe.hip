@@ -1,6 +1,7 @@
 // probe.hip -- roofline calibration: the encoder's memory traffic without its
 // arithmetic.  Lane i reads the four 16-byte rows of block i exactly as
-// k_encode_plain does (same tiles, same streaming hints) and writes 8 bytes
+// k_encode_plain does with its default launch shape (same tiles and
+// workgroups of 8 waves, same streaming hints) and writes 8 bytes
 // (an XOR fold of the 64), so its duration is the floor the encode kernels
 // can reach on a given device; bench.py reports the encoder against it.
 #include <hip/hip_ext.h>
@@ -49,7 +50,7 @@ extern "C" int goofy_hip_traffic_probe(void* d_out, const void* d_input, unsigne
         return fail(GOOFY_ERR_INVALID, "bad buffers or pitch");
     const uint32_t bw = width / 4, nblocks = bw * (height / 4);
     if (!magic_ok(bw, nblocks)) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
-    const dim3 grid((nblocks + 127u) / 128u), block(128);
+    const dim3 grid((nblocks + 511u) / 512u), block(512);  // the encoder's default workgroup of 8 waves
     hipExtLaunchKernelGGL(k_traffic_probe, grid, block, 0, (hipStream_t)stream, (hipEvent_t)start_event,
                           (hipEvent_t)stop_event, 0, (const uint8_t*)d_input, (uint64_t)pitch, (uint8_t*)d_out, bw,
                           nblocks, div_magic(bw));

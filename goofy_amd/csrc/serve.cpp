@@ -124,6 +124,7 @@ struct Slot {
 struct goofy_server {
     goofy_server_config cfg{};
     LaunchCfg launch{};
+    bool auto_shape = true;  // no shape configured: auto_cfg per launch
     int device = 0, num_cus = 0;
     Clock::time_point t0;
     Arena arena;
@@ -190,7 +191,13 @@ bool goofy_server::launch_group(Slot& s, std::vector<Req>& group)
         InlineWorklist wl;
         TexDesc* descs = inline_list ? wl.d : s.h_desc;
         uint32_t* begins = inline_list ? wl.tile_begin : worklist_tile_begin(s.h_desc, group.size());
-        const uint64_t tile = (uint64_t)tile_blocks(launch);
+        LaunchCfg L = launch;
+        if (auto_shape) {  // the shape follows the group's size
+            uint64_t blocks = 0;
+            for (const Req& q : group) blocks += (uint64_t)(q.r.width / 4) * (q.r.height / 4);
+            L = auto_cfg(blocks, num_cus);
+        }
+        const uint64_t tile = (uint64_t)tile_blocks(L);
         uint32_t tiles = 0;
         for (size_t i = 0; i < group.size(); ++i) {
             const goofy_request& r = group[i].r;
@@ -209,16 +216,16 @@ bool goofy_server::launch_group(Slot& s, std::vector<Req>& group)
         }
         begins[group.size()] = tiles;
         if (inline_list) {
-            e = launch_batch_inline(launch, wl, (uint32_t)group.size(), tiles, num_cus, s.stream);
+            e = launch_batch_inline(L, wl, (uint32_t)group.size(), tiles, num_cus, s.stream);
         } else {
             e = hipMemcpyAsync(s.d_desc, s.h_desc, worklist_bytes(group.size()), hipMemcpyHostToDevice, s.stream);
-            if (e == hipSuccess) e = launch_batch(launch, s.d_desc, (uint32_t)group.size(), tiles, num_cus, s.stream);
+            if (e == hipSuccess) e = launch_batch(L, s.d_desc, (uint32_t)group.size(), tiles, num_cus, s.stream);
         }
     } else {
         for (Req& q : group) {
-            const uint32_t bw = q.r.width / 4;
-            e = launch_plain(q.r.format, launch, q.r.src, q.r.pitch, arena.base() + q.arena_off, bw,
-                             bw * (q.r.height / 4), s.stream);
+            const uint32_t bw = q.r.width / 4, nb = bw * (q.r.height / 4);
+            e = launch_plain(q.r.format, auto_shape ? auto_cfg(nb, num_cus) : launch, q.r.src, q.r.pitch,
+                             arena.base() + q.arena_off, bw, nb, s.stream);
             if (e != hipSuccess) break;
         }
     }
@@ -353,9 +360,10 @@ int goofy_server_create(const goofy_server_config* ucfg, goofy_server** out)
     if (c.max_batch_textures <= 0) c.max_batch_textures = 256;
     if (c.max_pending <= 0) c.max_pending = 65536;
     if (c.streams > 64 || c.max_batch_textures > 65536) return fail(GOOFY_ERR_INVALID, "streams <= 64, batch <= 65536");
-    LaunchCfg lc = default_batch_cfg();
-    if (c.launch.blocks_per_lane || c.launch.wg_waves || c.launch.waves_per_cu || c.launch.nontemporal ||
-        c.launch.tile_order) {
+    LaunchCfg lc{1, 8, 0, 1, 0};
+    const bool auto_shape = !(c.launch.blocks_per_lane || c.launch.wg_waves || c.launch.waves_per_cu ||
+                              c.launch.nontemporal || c.launch.tile_order);
+    if (!auto_shape) {
         lc = to_cfg(c.launch);
         if (!cfg_valid(lc)) return fail(GOOFY_ERR_INVALID, "invalid launch config");
     }
@@ -365,6 +373,7 @@ int goofy_server_create(const goofy_server_config* ucfg, goofy_server** out)
     goofy_server* s = new goofy_server();
     s->cfg = c;
     s->launch = lc;
+    s->auto_shape = auto_shape;
     s->device = di.device;
     s->num_cus = di.num_cus;
     s->t0 = Clock::now();

@@ -81,7 +81,10 @@ typedef struct goofy_launch_config {
                             eighth of the tiles */
 } goofy_launch_config;
 
-/* As goofy_hip_compress_device with an explicit launch shape (NULL = default). */
+/* As goofy_hip_compress_device with an explicit launch shape.  NULL = the
+ * default shape, sized to the launch: one block per lane, streaming loads and
+ * stores, linear tile order, and the largest workgroup of 8, 4, 2 or 1 waves
+ * that still gives at least two workgroups per CU. */
 GOOFY_HIP_API int goofy_hip_compress_device_cfg(int format, void* d_result, const void* d_input, unsigned int width,
                                                 unsigned int height, size_t pitch, void* stream,
                                                 const goofy_launch_config* cfg);
@@ -177,8 +180,10 @@ typedef struct goofy_texture_desc {
 
 typedef struct goofy_batch goofy_batch;
 
-/* cfg NULL = library default; the plan keeps its own device copy of the
- * descriptors (the textures themselves stay owned by the caller). */
+/* cfg NULL = the tuned shape recorded for a batch like this one (see
+ * goofy_batch_autotune), else the default shape sized to the whole batch.
+ * The plan keeps its own device copy of the descriptors (the textures
+ * themselves stay owned by the caller). */
 GOOFY_HIP_API int goofy_batch_create(const goofy_texture_desc* textures, size_t count,
                                      const goofy_launch_config* cfg, goofy_batch** out);
 /* Enqueue the whole batch on `stream` (replays the captured graph). */
@@ -213,7 +218,7 @@ typedef struct goofy_server_config {
     int max_batch_textures;     /* textures per launch (0 = 256) */
     int max_pending;            /* queued, not yet admitted requests (0 = 65536) */
     int batching;               /* 1 = one launch per admitted group, 0 = one plain launch per request */
-    goofy_launch_config launch; /* kernel shape (all zero = library default) */
+    goofy_launch_config launch; /* kernel shape (all zero = the default shape, sized per launch) */
 } goofy_server_config;
 
 typedef struct goofy_request {
