@@ -109,14 +109,19 @@ def bench_plain(g, hip, W, steps: int, warmup: int, rotate: int, cfg: dict | Non
     kern_ms = {f: float(np.mean([a.elapsed_ms(b) for a, b in events[f]])) for f in events}
     kern_med = {f: float(np.median([a.elapsed_ms(b) for a, b in events[f]])) for f in events}
     kern_ms["single_stream_ms_per_step"] = elapsed_single / steps * 1e3
-    # the same traffic without the encode arithmetic: the attainable floor
-    probe = []
-    for i in range(2 * steps):
-        a, b = hip.Event(), hip.Event()
-        g.traffic_probe(outs[W.DXT1][i % rotate], srcs[i % rotate], im.width, im.height, im.pitch, a, b, stream)
-        probe.append((a, b))
-    stream.synchronize()
-    kern_ms["probe"] = float(np.mean([a.elapsed_ms(b) for a, b in probe[steps // 2:]]))
+    # the same traffic without the encode arithmetic: the attainable floor,
+    # best over workgroup sizes
+    kern_ms["probe_by_wg"] = {}
+    for wg in (1, 2, 4, 8):
+        probe = []
+        for i in range(steps):
+            a, b = hip.Event(), hip.Event()
+            g.traffic_probe(outs[W.DXT1][i % rotate], srcs[i % rotate], im.width, im.height, im.pitch, a, b, stream,
+                            wg_waves=wg)
+            probe.append((a, b))
+        stream.synchronize()
+        kern_ms["probe_by_wg"][wg] = float(np.mean([a.elapsed_ms(b) for a, b in probe[steps // 4:]]))
+    kern_ms["probe"] = min(kern_ms["probe_by_wg"].values())
     for b in srcs + outs[W.DXT1] + outs[W.ETC1]:
         b.free()
     stream.destroy()
@@ -484,7 +489,9 @@ def main() -> int:
                 "traffic": (traffic or {}).get("bytes_per_launch", {}).get("dxt1" if dom == W.DXT1 else "etc1"),
                 "kernel": "encode ETC1S" if dom == W.ETC1 else "encode DXT1",
                 "algorithmic_bytes_per_launch": algo,
-                "attainable_gbps": round(attainable, 1), "frac_of_attainable": round(achieved / attainable, 4)}
+                "attainable_gbps": round(attainable, 1), "frac_of_attainable": round(achieved / attainable, 4),
+                "probe_gbps_by_wg_waves": {str(k): round(algo / (v / 1e3) / 1e9, 1)
+                                           for k, v in kern_ms["probe_by_wg"].items()}}
     formats = {}
     for f, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
         formats[key] = {"kernel_ms_mean": round(kern_ms[f], 5), "kernel_ms_median": round(kern_med[f], 5),

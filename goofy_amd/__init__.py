@@ -274,13 +274,13 @@ def rgb565_baseline_device(dst, src, width: int, height: int, dst_pitch: int | N
             start_event.handle if start_event else None, stop_event.handle if stop_event else None))
 
 
-_sig("goofy_hip_traffic_probe", ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp, _vp, _vp)
+_sig("goofy_hip_traffic_probe", ctypes.c_int, _vp, _vp, _uint, _uint, _sz, ctypes.c_int, _vp, _vp, _vp)
 
 
 def traffic_probe(dst, src, width: int, height: int, pitch: int, start_event=None, stop_event=None,
-                  stream=None) -> None:
+                  stream=None, wg_waves: int = 0) -> None:
     """The encode kernel's memory traffic without its arithmetic (roofline calibration)."""
-    _check(_lib.goofy_hip_traffic_probe(_ptr(dst), _ptr(src), width, height, pitch, _stream(stream),
+    _check(_lib.goofy_hip_traffic_probe(_ptr(dst), _ptr(src), width, height, pitch, wg_waves, _stream(stream),
                                         start_event.handle if start_event else None,
                                         stop_event.handle if stop_event else None))
 

@@ -1,9 +1,9 @@
 // probe.hip -- roofline calibration: the encoder's memory traffic without its
 // arithmetic.  Lane i reads the four 16-byte rows of block i exactly as
-// k_encode_plain does with its default launch shape (same tiles and
-// workgroups of 8 waves, same streaming hints) and writes 8 bytes
-// (an XOR fold of the 64), so its duration is the floor the encode kernels
-// can reach on a given device; bench.py reports the encoder against it.
+// k_encode_plain does with one block per lane (same tiles, same streaming
+// hints, workgroups of 1-8 waves) and writes 8 bytes (an XOR fold of the 64),
+// so its best duration over the workgroup sizes is the floor the encode
+// kernels can reach on a given device; bench.py reports the encoder against it.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -39,9 +39,12 @@ __global__ __launch_bounds__(512) void k_traffic_probe(const uint8_t* __restrict
 using namespace goofy_impl;
 
 extern "C" int goofy_hip_traffic_probe(void* d_out, const void* d_input, unsigned int width, unsigned int height,
-                                       size_t pitch, void* stream, void* start_event, void* stop_event)
+                                       size_t pitch, int wg_waves, void* stream, void* start_event, void* stop_event)
 {
     clear_error();
+    if (wg_waves == 0) wg_waves = 8;
+    if (wg_waves != 1 && wg_waves != 2 && wg_waves != 4 && wg_waves != 8)
+        return fail(GOOFY_ERR_INVALID, "wg_waves must be 1, 2, 4 or 8");
     if (width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
     if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
     if (width == 0 || height == 0) return GOOFY_OK;
@@ -50,7 +53,8 @@ extern "C" int goofy_hip_traffic_probe(void* d_out, const void* d_input, unsigne
         return fail(GOOFY_ERR_INVALID, "bad buffers or pitch");
     const uint32_t bw = width / 4, nblocks = bw * (height / 4);
     if (!magic_ok(bw, nblocks)) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
-    const dim3 grid((nblocks + 511u) / 512u), block(512);  // the encoder's default workgroup of 8 waves
+    const uint32_t threads = 64u * (uint32_t)wg_waves;
+    const dim3 grid((nblocks + threads - 1u) / threads), block(threads);
     hipExtLaunchKernelGGL(k_traffic_probe, grid, block, 0, (hipStream_t)stream, (hipEvent_t)start_event,
                           (hipEvent_t)stop_event, 0, (const uint8_t*)d_input, (uint64_t)pitch, (uint8_t*)d_out, bw,
                           nblocks, div_magic(bw));

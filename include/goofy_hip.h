@@ -159,11 +159,14 @@ GOOFY_HIP_API int goofy_hip_rgb565_baseline_device_timed(void* d_dst, size_t dst
                                                          void* stream, void* start_event, void* stop_event);
 
 /* Roofline calibration: the plain encode kernel's memory traffic (same
- * addresses, tiles and streaming hints; 64 bytes read and 8 written per
- * block) with no encode arithmetic.  Its time is the floor for the encode
- * kernels on the device at hand.  Events as in goofy_hip_compress_device_timed. */
+ * addresses, tiles of 64 blocks, one per wave, and streaming hints; 64 bytes
+ * read and 8 written per block) with no encode arithmetic, in workgroups of
+ * wg_waves waves (1, 2, 4 or 8; 0 = 8).  Its best time over the workgroup
+ * sizes is the floor for the encode kernels on the device at hand.  Events as
+ * in goofy_hip_compress_device_timed. */
 GOOFY_HIP_API int goofy_hip_traffic_probe(void* d_out, const void* d_input, unsigned int width, unsigned int height,
-                                          size_t pitch, void* stream, void* start_event, void* stop_event);
+                                          size_t pitch, int wg_waves, void* stream, void* start_event,
+                                          void* stop_event);
 
 /* ---- batched encode: many textures, one launch -----------------------------
  * A plan flattens the block work-lists of n device-resident textures (each
