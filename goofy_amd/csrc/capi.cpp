@@ -490,7 +490,7 @@ static int run_quality(uint64_t pixels, uint64_t units, hipStream_t stream, goof
     goofy_quality q{};
     q.pixels = pixels;
     for (int i = 0; i < 4; ++i) q.sse_byte[i] = h.e[i];
-    q.sse_luma = h.y;
+    q.sse_luma = (double)h.y100 / 10000.0;
     const double n = (double)q.pixels;
     // labels as in the reference harness: "r" = byte 2, "b" = byte 0
     q.mse_r = (double)q.sse_byte[2] / n;

@@ -151,33 +151,31 @@ __global__ __launch_bounds__(256) void k_decode(const uint2* __restrict__ blocks
     }
 }
 
-// Squared error of one pixel pair into e[0..3] (bytes 0..3) and ey (luma
-// with the harness's weights, 0.21 / 0.72 / 0.07 on bytes 2 / 1 / 0; products
-// and sums kept unfused so each pixel's term is the one the harness forms).
-__device__ __forceinline__ void accum_pixel(uint32_t a, uint32_t d, uint32_t (&e)[4], double& ey)
+// Squared error of one pixel pair into e[0..3] (bytes 0..3) and ey (luma).
+// The harness's luma weights are 0.21 / 0.72 / 0.07 on bytes 2 / 1 / 0, so
+// 100 x the luma difference is the integer 21 dr + 72 dg + 7 db (|.| <= 25500);
+// ey sums its square exactly and the host divides by 10^4 once.  That is the
+// exact value of the harness's sum, which its own double arithmetic rounds
+// per pixel (relative differences of order 1e-15 per term).
+__device__ __forceinline__ void accum_pixel(uint32_t a, uint32_t d, uint32_t (&e)[4], uint64_t& ey)
 {
-#pragma clang fp contract(off)
+    int32_t diff[4];
 #pragma unroll
     for (int ch = 0; ch < 4; ++ch) {
-        const int32_t diff = (int32_t)((a >> (8 * ch)) & 0xFFu) - (int32_t)((d >> (8 * ch)) & 0xFFu);
-        e[ch] += (uint32_t)(diff * diff);
+        diff[ch] = (int32_t)((a >> (8 * ch)) & 0xFFu) - (int32_t)((d >> (8 * ch)) & 0xFFu);
+        e[ch] += (uint32_t)(diff[ch] * diff[ch]);
     }
-    const double y1 =
-        0.21 * (double)((a >> 16) & 0xFFu) + 0.72 * (double)((a >> 8) & 0xFFu) + 0.07 * (double)(a & 0xFFu);
-    const double y2 =
-        0.21 * (double)((d >> 16) & 0xFFu) + 0.72 * (double)((d >> 8) & 0xFFu) + 0.07 * (double)(d & 0xFFu);
-    const double dy = y2 - y1;
-    ey += dy * dy;
+    const int32_t dy100 = 21 * diff[2] + 72 * diff[1] + 7 * diff[0];
+    ey += (uint64_t)(uint32_t)(dy100 * dy100);  // <= 650,250,000
 }
 
 // Sums of one workgroup (256 lanes = 4 waves): wave reduction by lane
-// exchange, then the four wave sums in a fixed order, stored by lane 0 with a
-// plain vector store.  Every lane of the workgroup must reach this.  Fixed
-// orders everywhere make the double luma sum the same on every run.
-__device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], double ey, QualitySums* out)
+// exchange, then the four wave sums, stored by lane 0 with a plain vector
+// store.  Every lane of the workgroup must reach this.
+__device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], unsigned long long ey, QualitySums* out)
 {
     __shared__ unsigned long long s_e[4][4];
-    __shared__ double s_y[4];
+    __shared__ unsigned long long s_y[4];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -195,15 +193,15 @@ __device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], dou
         QualitySums r;
 #pragma unroll
         for (int ch = 0; ch < 4; ++ch) r.e[ch] = s_e[0][ch] + s_e[1][ch] + s_e[2][ch] + s_e[3][ch];
-        r.y = ((s_y[0] + s_y[1]) + s_y[2]) + s_y[3];
+        r.y100 = s_y[0] + s_y[1] + s_y[2] + s_y[3];
         *out = r;
     }
 }
 
 // Round-trip error: decode each block in registers and compare it with the
 // source pixels.  Per workgroup: e[0..3] = squared error of bytes 0..3 of the
-// pixels, y = sum of squared luma differences with the harness's weights
-// (0.21, 0.72, 0.07 on bytes 2, 1, 0 -- getMsePsnr reads a pixel as a
+// pixels, y100 = sum of squared luma differences x 10^4 with the harness's
+// weights (0.21, 0.72, 0.07 on bytes 2, 1, 0 -- getMsePsnr reads a pixel as a
 // little-endian word and takes bits 16..23 as "red").  Grid-stride over the
 // blocks, one partial per workgroup; k_sums_final adds the partials.
 template <int FMT>
@@ -211,9 +209,8 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint2* __restrict__ blocks,
                                                uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
                                                QualitySums* __restrict__ part)
 {
-#pragma clang fp contract(off)
     unsigned long long acc[4] = {0, 0, 0, 0};
-    double ey = 0.0;
+    uint64_t ey = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < nblocks; i += (uint64_t)gridDim.x * 256u) {
         const uint32_t by = (uint32_t)((i * magic) >> 40), bx = (uint32_t)i - by * bw;
         const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
@@ -240,9 +237,8 @@ __global__ __launch_bounds__(256) void k_sqerr_images(const uint8_t* __restrict_
                                                       uint32_t nquads, uint64_t magic,
                                                       QualitySums* __restrict__ part)
 {
-#pragma clang fp contract(off)
     unsigned long long acc[4] = {0, 0, 0, 0};
-    double ey = 0.0;
+    uint64_t ey = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < nquads; i += (uint64_t)gridDim.x * 256u) {
         const uint32_t y = (uint32_t)((i * magic) >> 40), x = (uint32_t)i - y * qw;
         const u32x4 va =
@@ -264,13 +260,12 @@ __global__ __launch_bounds__(256) void k_sqerr_images(const uint8_t* __restrict_
 __global__ __launch_bounds__(256) void k_sums_final(const QualitySums* __restrict__ part, uint32_t g,
                                                     QualitySums* __restrict__ out)
 {
-#pragma clang fp contract(off)
     unsigned long long acc[4] = {0, 0, 0, 0};
-    double ey = 0.0;
+    uint64_t ey = 0;
     for (uint32_t k = threadIdx.x; k < g; k += 256u) {
 #pragma unroll
         for (int ch = 0; ch < 4; ++ch) acc[ch] += part[k].e[ch];
-        ey += part[k].y;
+        ey += part[k].y100;
     }
     block_sums_store(acc, ey, out);
 }

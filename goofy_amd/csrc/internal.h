@@ -84,13 +84,14 @@ hipError_t launch_synth(const void* params, void* dst, uint32_t width, uint32_t 
                         hipStream_t stream);
 hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width, uint32_t height, size_t pitch,
                          hipStream_t stream);
-// Quality sums: squared errors of bytes 0..3 and the double luma sum.  The
+// Quality sums: squared errors of bytes 0..3, and the sum of squared luma
+// differences times 10^4 (an exact integer with the harness's weights).  The
 // quality launchers take a device scratch buffer of quality_scratch_bytes():
 // the final sums at offset 0, one partial per workgroup from
 // kQualityPartialsOffset on.
 struct QualitySums {
     unsigned long long e[4];
-    double y;
+    unsigned long long y100;
 };
 constexpr size_t kQualityPartialsOffset = 64;
 constexpr uint32_t kQualityMaxGroups = 2048;
