@@ -112,6 +112,15 @@ def test_floor_and_image_quality_validation_without_device():
     assert e.value.status == goofy_amd.ERR_WIDTH
 
 
+def test_probe_validation_without_device():
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.traffic_probe(8, 16, 16, 4, 64, wg_waves=3)
+    assert e.value.status == goofy_amd.ERR_INVALID and "wg_waves" in goofy_amd.last_error()
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.traffic_probe(8, 16, 18, 4, 80)
+    assert e.value.status == goofy_amd.ERR_WIDTH
+
+
 def test_batch_validation_without_device():
     bad = [dict(src=16, dst=8, width=6, height=4, pitch=32, format=0)]
     with pytest.raises(goofy_amd.GoofyError) as e:
