@@ -468,20 +468,69 @@ static double psnr_of(double mse, double peak)
     return 10.0 * std::log10((peak * peak) / mse);
 }
 
+// Device scratch of the quality sums, kept per device for reuse.  A call
+// returns its buffer only after synchronising its stream, so a buffer is never
+// shared by two calls in flight; no call frees one, because hipFree would wait
+// for every stream on the device (a quality check during serving would stall
+// the server's streams).
+class QualityScratchPool {
+public:
+    static constexpr size_t kBytes = kQualityPartialsOffset + (size_t)kQualityMaxGroups * sizeof(QualitySums);
+
+    void* acquire(int device)
+    {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            for (size_t i = 0; i < free_.size(); ++i)
+                if (free_[i].first == device) {
+                    void* p = free_[i].second;
+                    free_.erase(free_.begin() + (long)i);
+                    return p;
+                }
+        }
+        void* p = nullptr;
+        return hipMalloc(&p, kBytes) == hipSuccess ? p : nullptr;
+    }
+    void release(int device, void* p)
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        free_.emplace_back(device, p);
+    }
+
+private:
+    std::mutex mu_;
+    std::vector<std::pair<int, void*>> free_;
+};
+
+static QualityScratchPool& quality_scratch_pool()
+{
+    static QualityScratchPool* pool = new QualityScratchPool();  // never destroyed
+    return *pool;
+}
+
 // Runs `launch(scratch)` -- a quality launcher (QualitySums at offset 0 of
 // its scratch buffer when done) -- reads the sums back on `stream` and fills
 // `out` with getMsePsnr's figures (Src/main.cpp:403-469).
 extern "C++" template <typename Launch>
 static int run_quality(uint64_t pixels, uint64_t units, hipStream_t stream, goofy_quality* out, Launch&& launch)
 {
-    void* d = nullptr;
-    hipError_t e = hipMalloc(&d, quality_scratch_bytes(units));
-    if (e != hipSuccess) return fail_hip(e, "hipMalloc(quality sums)");
-    struct Free {
+    if (quality_scratch_bytes(units) > QualityScratchPool::kBytes) return fail(GOOFY_ERR_TOO_LARGE, "quality grid");
+    DeviceInfo di;
+    int rc = device_info(&di);
+    if (rc != GOOFY_OK) return rc;
+    void* d = quality_scratch_pool().acquire(di.device);
+    if (!d) return fail(GOOFY_ERR_OUT_OF_MEMORY, "quality scratch");
+    struct Release {  // on every path, the stream has drained before the buffer is reused
+        int dev;
         void* p;
-        ~Free() { (void)hipFree(p); }
-    } guard{d};
-    e = launch(d);
+        hipStream_t s;
+        ~Release()
+        {
+            (void)hipStreamSynchronize(s);
+            quality_scratch_pool().release(dev, p);
+        }
+    } guard{di.device, d, stream};
+    hipError_t e = launch(d);
     if (e != hipSuccess) return fail_hip(e, "quality kernel launch");
     QualitySums h;
     e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, stream);

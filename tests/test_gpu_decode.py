@@ -163,3 +163,44 @@ def test_quality_images_closed_form(gpu):
     assert q["mse_g"] == 4.0 and q["mse_rgb"] == 4.0
     assert q["mse_y"] == pytest.approx((0.72 * 22 - 0.72 * 20) ** 2, rel=1e-12)
     assert q["psnr_rgb"] == pytest.approx(10 * math.log10(768.0 ** 2 / 4.0), abs=1e-12)
+
+
+def test_quality_calls_from_threads_share_the_scratch_pool(gpu):
+    """Quality calls on four threads and streams at once (the library keeps a
+    pool of scratch buffers): every result equals the same call made alone."""
+    import threading
+
+    from goofy_amd import hip
+    rng = np.random.default_rng(3)
+    w, h = 256, 64
+    srcs, blocks = [], []
+    for i in range(4):
+        img = rng.integers(0, 256, w * h * 4, dtype=np.uint8)
+        srcs.append(hip.DeviceBuffer.from_array(img))
+        b = hip.DeviceBuffer((w // 4) * (h // 4) * 8)
+        gpu.compress_device(i % 2, b, srcs[-1], w, h, w * 4)
+        blocks.append(b)
+    hip.synchronize()
+    alone = [gpu.quality_device(i % 2, blocks[i], srcs[i], w, h) for i in range(4)]
+    got = [[] for _ in range(4)]
+    errors = []
+
+    def worker(i):
+        try:
+            s = hip.Stream()
+            for _ in range(20):
+                got[i].append(gpu.quality_device(i % 2, blocks[i], srcs[i], w, h, stream=s))
+            s.synchronize()
+            s.destroy()
+        except Exception as e:  # reported below, on the main thread
+            errors.append(e)
+
+    print("quality: 4 threads x 20 calls", flush=True)
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for i in range(4):
+        assert all(q == alone[i] for q in got[i]), i
