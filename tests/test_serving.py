@@ -63,6 +63,20 @@ def test_alone_equals_mix_plain_mode_and_tight_budget(gpu, pool):
         assert res["peak_inflight_bytes"] <= biggest * 2
 
 
+@pytest.mark.parametrize("launch", [dict(blocks_per_lane=2, wg_waves=4, waves_per_cu=4, nontemporal=1, tile_order=1),
+                                    dict(blocks_per_lane=4, wg_waves=1, waves_per_cu=0, nontemporal=0, tile_order=0)])
+def test_alone_equals_mix_with_explicit_launch_shapes(gpu, pool, launch):
+    """A server configured with a fixed shape (persistent grid and XCD order,
+    or 4 blocks per lane) instead of the per-launch default."""
+    from goofy_amd import serving
+    ts, p, expected = pool
+    say(f"serving: closed loop with launch {launch}")
+    res = serving.simulate(p, 0.0, 0.3, seed=11, closed_loop_depth=96, expected=expected, verify_every=1,
+                           server=serving.server_config(launch=launch))
+    assert res["completed"] == res["submitted"] > 50
+    assert res["mismatches"] == 0 and res["verified"] == res["completed"]
+
+
 def test_budget_admission_and_backpressure(gpu, pool):
     from goofy_amd import hip, serving
     import goofy_amd as g
