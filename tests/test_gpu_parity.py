@@ -334,43 +334,15 @@ def test_device_image_beyond_4gib(gpu, oracle):
 
 # ---- random blocks against the reference encoder itself ---------------------------------------
 
-def random_block_image(kind: str, rng, bw: int, bh: int) -> np.ndarray:
-    """An image of bw x bh independent random 4x4 blocks of one kind, as a
-    64-byte-aligned flat RGBA8 buffer with pitch 16 * bw."""
-    from oracle.oracle import aligned_empty
-    n = bw * bh
-    if kind == "uniform":
-        px = rng.integers(0, 256, (n, 16, 4))
-    elif kind == "narrow":  # small spreads around a random colour: every range 0..48
-        base = rng.integers(0, 256, (n, 1, 4))
-        spread = rng.integers(0, 25, (n, 1, 1))
-        px = base + rng.integers(-24, 25, (n, 16, 4)) * spread // 24
-    elif kind == "ramps":  # per-channel gradients with a little noise
-        t = np.linspace(0.0, 1.0, 16)[None, :, None]
-        a, b = rng.integers(0, 256, (n, 1, 4)), rng.integers(0, 256, (n, 1, 4))
-        px = (a + (b - a) * t).astype(np.int64) + rng.integers(-2, 3, (n, 16, 4))
-    else:  # "threshold": grey two-level and spread blocks whose lumas sit at the
-        # near/far and >=mid boundaries for every range
-        lo = rng.integers(0, 256, (n, 1))
-        r = rng.integers(0, 256, (n, 1)) * (255 - lo) // 255  # lo + r <= 255
-        frac = rng.integers(0, 9, (n, 16))  # positions in eighths of the range
-        g = lo + (r * frac) // 8
-        px = np.repeat(g[..., None], 4, axis=2)
-        px[..., 3] = rng.integers(0, 256, (n, 16))
-    px = np.clip(px, 0, 255).astype(np.uint8).reshape(bh, bw, 4, 4, 4)  # block row, block, y, x, ch
-    img = aligned_empty(n * 64)
-    img[:] = px.transpose(0, 2, 1, 3, 4).reshape(-1)
-    return img
-
-
 @pytest.mark.parametrize("kind", ["uniform", "narrow", "ramps", "threshold"])
 def test_random_blocks_vs_reference(gpu, reference, kind):
-    """262,144 random blocks per kind, both formats, bit-exact against the
-    reference encoder compiled from its own source (oracle/_ref)."""
-    rng = np.random.default_rng({"uniform": 1, "narrow": 2, "ramps": 3, "threshold": 4}[kind])
-    bw, bh = 1024, 256
+    """2,621,440 random blocks per kind (10.5M over the four kinds, see
+    tests/blockgen.py), both formats, bit-exact against the reference encoder
+    compiled from its own source (oracle/_ref)."""
+    from blockgen import block_image
+    bw, bh = 4096, 640
     w, h = 4 * bw, 4 * bh
-    img = random_block_image(kind, rng, bw, bh)
+    img = block_image(kind, {"uniform": 1, "narrow": 2, "ramps": 3, "threshold": 4}[kind], bw, bh)
     src = upload(img)
     for fmt in (W.DXT1, W.ETC1):
         say(f"random blocks {kind} fmt={fmt}: {bw * bh} blocks")
