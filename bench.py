@@ -7,13 +7,23 @@ go to two streams), so MP/s counts the pixels of both formats per second of
 wall time over the timed steps.  Four input buffers of 256 MiB are rotated so
 that no launch reads an image the 256 MB last-level cache still holds.
 
-Also reported, from a second timed region of the same steps on one stream:
-per-format kernel times (HIP events recorded by each kernel's own dispatch)
-and the HBM roofline of the slower kernel, beside a traffic probe (same
-addresses, no encode arithmetic) that gives the attainable floor; the batched workload (4096 textures,
-one persistent graph launch vs 4096 plain launches); the serving workload
-(Poisson arrivals, p50/p99 latency); and the reference encoder on the host
-CPU (compiled from its own source, one stripe of block rows per thread).
+Also reported:
+* from a second timed region of the same steps on one stream, per-format
+  kernel times (HIP events recorded by each kernel's own dispatch) and the HBM
+  roofline of the slower kernel, beside a traffic probe (same addresses, no
+  encode arithmetic, best of four workgroup sizes) that gives the attainable
+  floor;
+* the 16384^2 stress image with its padded pitch (default and tuned launch
+  shapes);
+* the decoders, the RGB565 quality floor and the quality sums on the 8192^2
+  image;
+* the batched workload: 4096 textures in one autotuned graph launch against
+  4096 plain launches;
+* the serving workload: Poisson arrivals, p50/p99 latency, and capacity
+  batched against one launch per request;
+* the drop-in on host buffers (PCIe included);
+* the reference encoder on the host CPU (compiled from its own source, one
+  stripe of block rows per thread).
 """
 from __future__ import annotations
 
