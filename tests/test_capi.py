@@ -30,6 +30,21 @@ def test_every_declared_symbol_is_exported():
     assert not missing, missing
 
 
+def test_headers_compile_as_c_and_cpp(tmp_path):
+    # include/goofy_hip.h is a C header (plain pointers and sizes); goofy_tc.h is
+    # the reference's C++ API; both must also compile side by side in one TU
+    c = tmp_path / "t.c"
+    c.write_text('#include "goofy_hip.h"\nint main(void) { goofy_launch_config k = {1, 8, 0, 1, 0};'
+                 ' goofy_quality q; (void)k; (void)q; return goofy_hip_device_count() < 0; }\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", f"-I{ROOT / 'include'}", str(c)],
+                   check=True)
+    cpp = tmp_path / "t.cpp"
+    cpp.write_text('#include "goofy_tc.h"\n#include "goofy_hip.h"\n'
+                   'int main() { return goofy::compressDXT1(nullptr, nullptr, 0, 0, 0); }\n')
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", f"-I{ROOT / 'include'}", str(cpp)],
+                   check=True)
+
+
 def test_dropin_cpp_api_exported():
     out = subprocess.run(["nm", "-D", "-C", "--defined-only", str(LIB)], capture_output=True, text=True,
                          check=True).stdout
