@@ -211,21 +211,39 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint2* __restrict__ blocks,
 {
     unsigned long long acc[4] = {0, 0, 0, 0};
     uint64_t ey = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < nblocks; i += (uint64_t)gridDim.x * 256u) {
+    const uint64_t step = (uint64_t)gridDim.x * 256u;
+    // Software pipeline: the loads of a lane's next block are issued before
+    // the current block is decoded and compared, so the reads stay in flight
+    // across the arithmetic.
+    auto fetch = [&](uint64_t i, uint2& b, u32x4 (&v)[4]) {
         const uint32_t by = (uint32_t)((i * magic) >> 40), bx = (uint32_t)i - by * bw;
         const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
+        b = load_block(blocks + i);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            v[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
+    };
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    uint2 nb = make_uint2(0u, 0u);
+    u32x4 nv[4] = {};
+    if (i < nblocks) fetch(i, nb, nv);
+    while (i < nblocks) {
+        const uint2 b = nb;
+        const u32x4 v[4] = {nv[0], nv[1], nv[2], nv[3]};
+        const uint64_t next = i + step;
+        if (next < nblocks) fetch(next, nb, nv);
         uint32_t px[16];
-        decode_block<FMT>(load_block(blocks + i), px);
+        decode_block<FMT>(b, px);
         uint32_t e[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
-            const uint32_t s[4] = {v.x, v.y, v.z, v.w};
+            const uint32_t s[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
 #pragma unroll
             for (int c = 0; c < 4; ++c) accum_pixel(s[c], px[4 * r + c], e, ey);
         }
 #pragma unroll
         for (int ch = 0; ch < 4; ++ch) acc[ch] += e[ch];
+        i = next;
     }
     block_sums_store(acc, ey, part + blockIdx.x);
 }

@@ -94,7 +94,7 @@ struct QualitySums {
     unsigned long long y100;
 };
 constexpr size_t kQualityPartialsOffset = 64;
-constexpr uint32_t kQualityMaxGroups = 2048;
+constexpr uint32_t kQualityMaxGroups = 1536;
 inline uint32_t quality_grid(uint64_t units)
 {
     const uint64_t g = (units + 255u) / 256u;

@@ -123,7 +123,7 @@ GOOFY_HIP_API int goofy_hip_decode_device(int format, void* d_rgba, const void* 
  * "b" = 0..7, "a" = 24..31; luma = 0.21 r + 0.72 g + 0.07 b; PSNR =
  * 10 log10(peak^2 / mse) with peak 255, and 768 for the summed-RGB figure;
  * an MSE below 1e-7 gives DBL_MAX.  Synchronous on `stream`.  Uses an
- * 80 KiB device scratch buffer from a per-device pool (allocated on first
+ * 60 KiB device scratch buffer from a per-device pool (allocated on first
  * use, never freed, one per concurrent call), so a call never waits on other
  * streams. */
 typedef struct goofy_quality {
