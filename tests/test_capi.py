@@ -164,3 +164,11 @@ def test_synth_host_deterministic_and_threaded():
     rows = c.reshape(70, 130 * 4 + 24)
     assert (rows[:, 130 * 4:] == 0xEE).all()          # padding untouched
     assert (rows[:, 3:130 * 4:4] == 255).all()          # alpha = 255
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # bench.py measures one GPU; --gpus N > 1 (or a multi-rank launch) exits with a message
+    import sys
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
