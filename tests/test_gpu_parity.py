@@ -143,6 +143,22 @@ def test_device_shapes_vs_oracle(gpu, oracle, seed):
         assert np.array_equal(encode_dev(gpu, fmt, src, w, h, pitch), want)
 
 
+@pytest.mark.parametrize("w,h", [(4, 4), (4, 16384), (16384, 4), (65532, 4), (4, 65532), (12, 8)])
+def test_extreme_aspect_ratios_vs_oracle(gpu, oracle, w, h):
+    # one block column, one block row, a single block, and rows of 16383 blocks:
+    # tiles that wrap across many block rows, or never do
+    rng = np.random.default_rng(w * 7 + h)
+    pitch = w * 4 + 16
+    img = rng.integers(0, 256, pitch * h, dtype=np.uint8)
+    src = upload(img)
+    for fmt in (W.DXT1, W.ETC1):
+        say(f"aspect {w}x{h} fmt={fmt}")
+        want = oracle.compress_blocks(fmt, img, w, h, pitch)
+        assert np.array_equal(encode_dev(gpu, fmt, src, w, h, pitch), want)
+        cfg = dict(blocks_per_lane=4, wg_waves=8, waves_per_cu=0, nontemporal=1, tile_order=1)
+        assert np.array_equal(encode_dev(gpu, fmt, src, w, h, pitch, cfg), want)
+
+
 @pytest.mark.parametrize("bpl", [1, 2, 4])
 @pytest.mark.parametrize("wg", [1, 2, 4, 8])
 @pytest.mark.parametrize("nt", [0, 1])
