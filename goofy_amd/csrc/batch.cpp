@@ -64,19 +64,26 @@ int build_worklist(goofy_batch* b)
     b->descs.clear();
     uint64_t tiles = 0;
     for (const goofy_texture_desc& t : b->textures) {
-        TexDesc d{};
-        const uint32_t bw = t.width / 4, bh = t.height / 4;
-        d.src = (uint64_t)(uintptr_t)t.src;
-        d.dst = (uint64_t)(uintptr_t)t.dst;
-        d.pitch = t.pitch;
-        d.bw = bw;
-        d.nblocks = bw * bh;
-        d.magic = div_magic(bw);
-        d.fmt = (uint32_t)t.format;
-        d.tile_begin = (uint32_t)tiles;
-        tiles += (d.nblocks + tile - 1) / tile;
-        if (tiles > 0xFFFFFFFFull / tile) return fail(GOOFY_ERR_TOO_LARGE, "batch has too many tiles");
-        b->descs.push_back(d);
+        // one entry per horizontal stripe of at most max_stripe_brows block
+        // rows (a single stripe unless the texture is beyond the kernel's
+        // index range)
+        const uint32_t bw = t.width / 4, bh = t.height / 4, max_rows = max_stripe_brows(bw);
+        for (uint32_t y0 = 0; y0 < bh;) {
+            const uint32_t rows = std::min(max_rows, bh - y0);
+            TexDesc d{};
+            d.src = (uint64_t)(uintptr_t)t.src + (uint64_t)y0 * 4 * t.pitch;
+            d.dst = (uint64_t)(uintptr_t)t.dst + (uint64_t)y0 * bw * 8;
+            d.pitch = t.pitch;
+            d.bw = bw;
+            d.nblocks = rows * bw;
+            d.magic = div_magic(bw);
+            d.fmt = (uint32_t)t.format;
+            d.tile_begin = (uint32_t)tiles;
+            tiles += (d.nblocks + tile - 1) / tile;
+            if (tiles > 0xFFFFFFFFull / tile) return fail(GOOFY_ERR_TOO_LARGE, "batch has too many tiles");
+            b->descs.push_back(d);
+            y0 += rows;
+        }
     }
     b->ntiles = (uint32_t)tiles;
     const size_t ntex = b->descs.size();
@@ -199,8 +206,8 @@ int check_texture(const goofy_texture_desc& t, size_t i)
         return fail(GOOFY_ERR_INVALID, std::string(where) + "pitch must be >= 4*width and a multiple of 16");
     if ((uintptr_t)t.src % 16 != 0 || (uintptr_t)t.dst % 8 != 0)
         return fail(GOOFY_ERR_INVALID, std::string(where) + "src must be 16-byte and dst 8-byte aligned");
-    const uint64_t bw = t.width / 4, nb = bw * (t.height / 4);
-    if (!magic_ok(bw, nb) || nb > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, std::string(where) + "too large");
+    if (max_stripe_brows(t.width / 4) == 0)  // larger textures become several work-list entries
+        return fail(GOOFY_ERR_TOO_LARGE, std::string(where) + "too wide (4,194,304 pixels or more per row)");
     return GOOFY_OK;
 }
 

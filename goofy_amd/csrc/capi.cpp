@@ -224,8 +224,8 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
     if (!result || !input) return fail(GOOFY_ERR_INVALID, "null buffer");
     if ((size_t)stride < (size_t)width * 4) return fail(GOOFY_ERR_INVALID, "stride smaller than 4*width");
     const uint32_t bw = width / 4, bh = height / 4;
-    const uint64_t nblocks = (uint64_t)bw * bh;
-    if (!magic_ok(bw, nblocks) || nblocks > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
+    const uint32_t max_rows = max_stripe_brows(bw);  // per chunk: chunks are launched one by one
+    if (max_rows == 0) return fail(GOOFY_ERR_TOO_LARGE, "image too wide (4,194,304 pixels or more per row)");
 
     DeviceInfo di;
     int rc = device_info(&di);
@@ -238,7 +238,7 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
 
     const size_t row_bytes = (size_t)width * 4;
     const unsigned chunk_brows = (unsigned)std::max<size_t>(1, (size_t(8) << 20) / (row_bytes * 4));  // ~8 MiB
-    const unsigned cb = std::min(chunk_brows, bh);
+    const unsigned cb = std::min(std::min(chunk_brows, bh), max_rows);
     if ((rc = ensure_ctx(ctx, row_bytes * 4 * cb, (size_t)bw * cb * 8)) != GOOFY_OK) return rc;
     const unsigned nchunks = (bh + cb - 1) / cb;
     const LaunchCfg cfg = auto_cfg((uint64_t)bw * cb, di.num_cus);
@@ -295,9 +295,10 @@ static int compress_device(int fmt, void* d_result, const void* d_input, unsigne
     if ((uintptr_t)d_input % 16 != 0 || (uintptr_t)d_result % 8 != 0)
         return fail(GOOFY_ERR_INVALID, "input must be 16-byte and result 8-byte aligned");
     const uint32_t bw = width / 4, bh = height / 4;
-    const uint64_t nblocks = (uint64_t)bw * bh;
-    if (!magic_ok(bw, nblocks) || nblocks > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
-    LaunchCfg cfg;
+    const uint32_t max_rows = max_stripe_brows(bw);
+    if (max_rows == 0) return fail(GOOFY_ERR_TOO_LARGE, "image too wide (4,194,304 pixels or more per row)");
+    LaunchCfg cfg{};
+    int num_cus = 0;
     if (ucfg) {
         cfg = to_cfg(*ucfg);
         if (!cfg_valid(cfg)) return fail(GOOFY_ERR_INVALID, "invalid launch config");
@@ -305,11 +306,22 @@ static int compress_device(int fmt, void* d_result, const void* d_input, unsigne
         DeviceInfo di;
         int rc = device_info(&di);
         if (rc != GOOFY_OK) return rc;
-        cfg = auto_cfg(nblocks, di.num_cus);
+        num_cus = di.num_cus;
     }
-    hipError_t e = launch_plain(fmt, cfg, d_input, pitch, d_result, bw, (uint32_t)nblocks, (hipStream_t)stream,
-                                (hipEvent_t)ev_start, (hipEvent_t)ev_stop);
-    if (e != hipSuccess) return fail_hip(e, "encode kernel launch");
+    // One launch, or horizontal stripes for images beyond one launch's index
+    // range; the start event goes with the first launch, the stop event with
+    // the last, so the interval covers them all.
+    for (uint32_t y0 = 0; y0 < bh;) {
+        const uint32_t rows = std::min(max_rows, bh - y0);
+        const uint32_t nb = rows * bw;  // <= 0xFFFFFFFF by max_stripe_brows
+        const LaunchCfg c = ucfg ? cfg : auto_cfg(nb, num_cus);
+        hipError_t e = launch_plain(fmt, c, static_cast<const uint8_t*>(d_input) + (size_t)y0 * 4 * pitch, pitch,
+                                    static_cast<uint8_t*>(d_result) + (size_t)y0 * bw * 8, bw, nb,
+                                    (hipStream_t)stream, y0 == 0 ? (hipEvent_t)ev_start : nullptr,
+                                    y0 + rows == bh ? (hipEvent_t)ev_stop : nullptr);
+        if (e != hipSuccess) return fail_hip(e, "encode kernel launch");
+        y0 += rows;
+    }
     return GOOFY_OK;
 }
 
@@ -434,8 +446,10 @@ int goofy_hip_tune_device(int format, void* d_result, const void* d_input, unsig
     return GOOFY_OK;
 }
 
+// striped = the caller launches stripes of max_stripe_brows block rows, so
+// only a single block row has to fit one launch's index range.
 static int check_image_device(int fmt, const void* d_img, const void* d_blocks, unsigned width, unsigned height,
-                              size_t pitch)
+                              size_t pitch, bool striped)
 {
     if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
     if (width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
@@ -447,7 +461,8 @@ static int check_image_device(int fmt, const void* d_img, const void* d_blocks, 
     if ((uintptr_t)d_img % 16 != 0 || (uintptr_t)d_blocks % 8 != 0)
         return fail(GOOFY_ERR_INVALID, "image must be 16-byte and blocks 8-byte aligned");
     const uint64_t bw = width / 4, nb = bw * (height / 4);
-    if (!magic_ok(bw, nb) || nb > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
+    if (striped ? max_stripe_brows(bw) == 0 : (!magic_ok(bw, nb) || nb > 0xFFFFFFFFull))
+        return fail(GOOFY_ERR_TOO_LARGE, "image too large");
     return GOOFY_OK;
 }
 
@@ -455,10 +470,17 @@ int goofy_hip_decode_device(int format, void* d_rgba, const void* d_blocks, unsi
                             size_t pitch, void* stream)
 {
     clear_error();
-    int rc = check_image_device(format, d_rgba, d_blocks, width, height, pitch);
+    int rc = check_image_device(format, d_rgba, d_blocks, width, height, pitch, true);
     if (rc != GOOFY_OK || width == 0 || height == 0) return rc;
-    hipError_t e = launch_decode(format, d_blocks, d_rgba, width, height, pitch, (hipStream_t)stream);
-    if (e != hipSuccess) return fail_hip(e, "decode kernel launch");
+    const uint32_t bw = width / 4, bh = height / 4, max_rows = max_stripe_brows(bw);
+    for (uint32_t y0 = 0; y0 < bh;) {  // stripes for images beyond one launch's index range
+        const uint32_t rows = std::min(max_rows, bh - y0);
+        hipError_t e = launch_decode(format, static_cast<const uint8_t*>(d_blocks) + (size_t)y0 * bw * 8,
+                                     static_cast<uint8_t*>(d_rgba) + (size_t)y0 * 4 * pitch, width, rows * 4, pitch,
+                                     (hipStream_t)stream);
+        if (e != hipSuccess) return fail_hip(e, "decode kernel launch");
+        y0 += rows;
+    }
     return GOOFY_OK;
 }
 
@@ -565,7 +587,7 @@ int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_sou
 {
     clear_error();
     if (!out) return fail(GOOFY_ERR_INVALID, "null output");
-    int rc = check_image_device(format, d_source, d_blocks, width, height, pitch);
+    int rc = check_image_device(format, d_source, d_blocks, width, height, pitch, false);
     if (rc != GOOFY_OK) return rc;
     if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
     const uint64_t nblocks = (uint64_t)(width / 4) * (height / 4);

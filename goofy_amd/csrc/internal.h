@@ -73,6 +73,21 @@ inline uint64_t div_magic(uint32_t bw) { return ((1ull << 40) + bw - 1) / bw; }
 // True when (i * magic) >> 40 == i / bw for every i < nblocks.
 inline bool magic_ok(uint64_t bw, uint64_t nblocks) { return bw > 0 && nblocks * bw < (1ull << 40) && nblocks / bw < (1ull << 24); }
 
+// Most block rows one launch (or one work-list entry) may cover for an image
+// bw blocks wide: block indices below 2^32 and an exact division magic
+// (magic_ok).  Larger images are encoded as horizontal stripes of at most
+// this many block rows; blocks are independent, so stripes change nothing in
+// the output.  0 when a single block row is already beyond the range
+// (bw >= 2^20, i.e. 4,194,304 pixels or more per row).
+inline uint32_t max_stripe_brows(uint64_t bw)
+{
+    if (bw == 0 || bw >= (1ull << 20)) return 0;
+    uint64_t r = ((1ull << 40) - 1) / (bw * bw);
+    if (r > 0xFFFFFFFFull / bw) r = 0xFFFFFFFFull / bw;
+    if (r > (1ull << 24) - 1) r = (1ull << 24) - 1;
+    return (uint32_t)r;
+}
+
 // Kernel launchers (kernels.hip).
 hipError_t launch_plain(int fmt, const LaunchCfg& cfg, const void* src, size_t pitch, void* dst, uint32_t bw,
                         uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);

@@ -127,6 +127,17 @@ def test_floor_and_image_quality_validation_without_device():
     assert e.value.status == goofy_amd.ERR_WIDTH
 
 
+def test_too_wide_images_refused_without_device():
+    # 4,194,304 pixels per row (2^20 blocks) is beyond one launch's index range
+    # even for a single block row; narrower images of any height are striped
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.compress_device(goofy_amd.DXT1, 8, 16, 4194304, 4, 4194304 * 4)
+    assert e.value.status == goofy_amd.ERR_TOO_LARGE
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.Batch([dict(src=16, dst=8, width=4194304, height=4, pitch=4194304 * 4, format=0)])
+    assert e.value.status == goofy_amd.ERR_TOO_LARGE
+
+
 def test_probe_validation_without_device():
     with pytest.raises(goofy_amd.GoofyError) as e:
         goofy_amd.traffic_probe(8, 16, 16, 4, 64, wg_waves=3)

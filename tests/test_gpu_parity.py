@@ -159,6 +159,38 @@ def test_extreme_aspect_ratios_vs_oracle(gpu, oracle, w, h):
         assert np.array_equal(encode_dev(gpu, fmt, src, w, h, pitch, cfg), want)
 
 
+def test_images_beyond_one_launch_are_striped(gpu, oracle):
+    """2,097,152 x 16 pixels: 2^19 blocks per row, so one launch covers at most
+    3 block rows (block index x blocks per row < 2^40).  Encode, decode and a
+    batch entry all run as stripes of 3 + 1 block rows, bit-exact."""
+    from goofy_amd import hip
+    w, h = 1 << 21, 16
+    pitch = w * 4
+    rng = np.random.default_rng(21)
+    img = rng.integers(0, 256, pitch * h, dtype=np.uint8)
+    src = upload(img)
+    for fmt in (W.DXT1, W.ETC1):
+        say(f"striped {w}x{h} fmt={fmt}")
+        want = oracle.compress_blocks(fmt, img, w, h, pitch)
+        got_buf = dev_buffer(want.size, fill=0xCD)
+        gpu.compress_device(fmt, got_buf, src, w, h, pitch)
+        assert np.array_equal(to_host(got_buf), want)
+        dec = dev_buffer(pitch * h, fill=0x3C)
+        gpu.decode_device(fmt, dec, got_buf, w, h, pitch)
+        assert np.array_equal(to_host(dec), oracle.decode_image(fmt, want, w, h))
+        got_buf.fill(0xCD)
+        small = dev_buffer(64 * 64 * 4, fill=7)
+        small_out = dev_buffer(16 * 16 * 8, fill=0xCD)
+        b = gpu.Batch([dict(src=src, dst=got_buf, width=w, height=h, pitch=pitch, format=fmt),
+                       dict(src=small, dst=small_out, width=64, height=64, pitch=256, format=fmt)])
+        b.launch()
+        hip.synchronize()
+        assert np.array_equal(to_host(got_buf), want)
+        assert np.array_equal(to_host(small_out),
+                              oracle.compress_blocks(fmt, np.full(64 * 64 * 4, 7, np.uint8), 64, 64, 256))
+        b.close()
+
+
 @pytest.mark.parametrize("bpl", [1, 2, 4])
 @pytest.mark.parametrize("wg", [1, 2, 4, 8])
 @pytest.mark.parametrize("nt", [0, 1])
