@@ -191,6 +191,29 @@ def test_images_beyond_one_launch_are_striped(gpu, oracle):
         b.close()
 
 
+def test_65536_wide_image_striped_vs_reference(gpu, reference):
+    """65536 x 16384 pixels (4 GiB, generated on the device): blocks x blocks per
+    row is just above 2^40, so the encode runs as 2 stripes.  Both formats,
+    all 67M blocks against the reference encoder."""
+    from goofy_amd import hip
+    from oracle.oracle import aligned_empty
+    w, h = 65536, 16384
+    pitch = w * 4
+    nb = (w // 4) * (h // 4)
+    say(f"wide striped {w}x{h}: synth on device")
+    src = dev_buffer(pitch * h)
+    gpu.synth_device(0, 0x601F000A, 0, 12, src, w, h, pitch)
+    img = aligned_empty(pitch * h)
+    hip.check(hip._hip.hipMemcpy(img.ctypes.data, src.data_ptr(), pitch * h, hip.D2H), "download")
+    out = dev_buffer(nb * 8, fill=0xCD)
+    for fmt in (W.DXT1, W.ETC1):
+        say(f"wide striped {w}x{h} fmt={fmt}")
+        gpu.compress_device(fmt, out, src, w, h, pitch)
+        want = reference.compress_mt(fmt, img, w, h, pitch, 16)
+        assert sha(to_host(out)) == sha(want), fmt
+    del img
+
+
 @pytest.mark.parametrize("bpl", [1, 2, 4])
 @pytest.mark.parametrize("wg", [1, 2, 4, 8])
 @pytest.mark.parametrize("nt", [0, 1])
