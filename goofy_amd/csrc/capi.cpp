@@ -587,12 +587,21 @@ int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_sou
 {
     clear_error();
     if (!out) return fail(GOOFY_ERR_INVALID, "null output");
-    int rc = check_image_device(format, d_source, d_blocks, width, height, pitch, false);
+    int rc = check_image_device(format, d_source, d_blocks, width, height, pitch, true);
     if (rc != GOOFY_OK) return rc;
     if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
-    const uint64_t nblocks = (uint64_t)(width / 4) * (height / 4);
-    return run_quality((uint64_t)width * height, nblocks, (hipStream_t)stream, out, [&](void* scratch) {
-        return launch_sqerr(format, d_blocks, d_source, pitch, width, height, scratch, (hipStream_t)stream);
+    const uint32_t bw = width / 4, bh = height / 4, max_rows = max_stripe_brows(bw);
+    const uint64_t stripe_blocks = (uint64_t)bw * std::min(max_rows, bh);
+    return run_quality((uint64_t)width * height, stripe_blocks, (hipStream_t)stream, out, [&](void* scratch) {
+        hipError_t e = hipSuccess;
+        for (uint32_t y0 = 0; y0 < bh && e == hipSuccess;) {  // stripes beyond one launch's index range
+            const uint32_t rows = std::min(max_rows, bh - y0);
+            e = launch_sqerr(format, static_cast<const uint8_t*>(d_blocks) + (size_t)y0 * bw * 8,
+                             static_cast<const uint8_t*>(d_source) + (size_t)y0 * 4 * pitch, pitch, width, rows * 4,
+                             scratch, y0 > 0, (hipStream_t)stream);
+            y0 += rows;
+        }
+        return e;
     });
 }
 
@@ -611,8 +620,8 @@ static int check_rgba_pair(const void* d_a, size_t pitch_a, const void* d_b, siz
         return fail(GOOFY_ERR_INVALID, "pitches must be >= 4*width and multiples of 16");
     if ((uintptr_t)d_a % 16 != 0 || (uintptr_t)d_b % 16 != 0)
         return fail(GOOFY_ERR_INVALID, "images must be 16-byte aligned");
-    const uint64_t qw = width / 4, n = qw * height;
-    if (!magic_ok(qw, n) || n > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "image too large");
+    if (max_stripe_brows(width / 4) == 0)  // larger images run as stripes of rows
+        return fail(GOOFY_ERR_TOO_LARGE, "image too wide (4,194,304 pixels or more per row)");
     return GOOFY_OK;
 }
 
@@ -624,9 +633,18 @@ int goofy_hip_quality_images_device(const void* d_a, size_t pitch_a, const void*
     int rc = check_rgba_pair(d_a, pitch_a, d_b, pitch_b, width, height, 1);
     if (rc != GOOFY_OK) return rc;
     if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
-    const uint64_t nquads = (uint64_t)(width / 4) * height;
-    return run_quality((uint64_t)width * height, nquads, (hipStream_t)stream, out, [&](void* scratch) {
-        return launch_sqerr_images(d_a, pitch_a, d_b, pitch_b, width, height, scratch, (hipStream_t)stream);
+    const uint32_t qw = width / 4, max_rows = max_stripe_brows(qw);  // rows of qw 4-pixel units
+    const uint64_t stripe_units = (uint64_t)qw * std::min(max_rows, height);
+    return run_quality((uint64_t)width * height, stripe_units, (hipStream_t)stream, out, [&](void* scratch) {
+        hipError_t e = hipSuccess;
+        for (uint32_t y0 = 0; y0 < height && e == hipSuccess;) {
+            const uint32_t rows = std::min(max_rows, height - y0);
+            e = launch_sqerr_images(static_cast<const uint8_t*>(d_a) + (size_t)y0 * pitch_a, pitch_a,
+                                    static_cast<const uint8_t*>(d_b) + (size_t)y0 * pitch_b, pitch_b, width, rows,
+                                    scratch, y0 > 0, (hipStream_t)stream);
+            y0 += rows;
+        }
+        return e;
     });
 }
 
@@ -644,9 +662,19 @@ int goofy_hip_rgb565_baseline_device_timed(void* d_dst, size_t dst_pitch, const 
     clear_error();
     int rc = check_rgba_pair(d_dst, dst_pitch, d_src, src_pitch, width, height, 2);
     if (rc != GOOFY_OK || width == 0 || height == 0) return rc;
-    hipError_t e = launch_rgb565_baseline(d_src, src_pitch, d_dst, dst_pitch, width, height, (hipStream_t)stream,
-                                          (hipEvent_t)start_event, (hipEvent_t)stop_event);
-    if (e != hipSuccess) return fail_hip(e, "rgb565 baseline kernel launch");
+    // the kernel's rows are pairs of pixel rows; stripes of at most
+    // max_stripe_brows such pairs keep every launch within its index range
+    const uint32_t max_px_rows = 2u * max_stripe_brows(width / 4);
+    for (uint32_t y0 = 0; y0 < height;) {
+        const uint32_t rows = std::min(max_px_rows, height - y0);
+        hipError_t e = launch_rgb565_baseline(static_cast<const uint8_t*>(d_src) + (size_t)y0 * src_pitch, src_pitch,
+                                              static_cast<uint8_t*>(d_dst) + (size_t)y0 * dst_pitch, dst_pitch,
+                                              width, rows, (hipStream_t)stream,
+                                              y0 == 0 ? (hipEvent_t)start_event : nullptr,
+                                              y0 + rows == height ? (hipEvent_t)stop_event : nullptr);
+        if (e != hipSuccess) return fail_hip(e, "rgb565 baseline kernel launch");
+        y0 += rows;
+    }
     return GOOFY_OK;
 }
 

@@ -274,12 +274,19 @@ __global__ __launch_bounds__(256) void k_sqerr_images(const uint8_t* __restrict_
     block_sums_store(acc, ey, part + blockIdx.x);
 }
 
-// One workgroup adds the g partials in a fixed order into *out.
+// One workgroup adds the g partials in a fixed order into *out; with
+// accumulate, onto the sums already in *out (the stripes of a large image,
+// launched one after another on one stream).
 __global__ __launch_bounds__(256) void k_sums_final(const QualitySums* __restrict__ part, uint32_t g,
-                                                    QualitySums* __restrict__ out)
+                                                    QualitySums* __restrict__ out, uint32_t accumulate)
 {
     unsigned long long acc[4] = {0, 0, 0, 0};
     uint64_t ey = 0;
+    if (accumulate && threadIdx.x == 0) {  // read before lane 0 writes the new sums at the end
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) acc[ch] = out->e[ch];
+        ey = out->y100;
+    }
     for (uint32_t k = threadIdx.x; k < g; k += 256u) {
 #pragma unroll
         for (int ch = 0; ch < 4; ++ch) acc[ch] += part[k].e[ch];
@@ -347,7 +354,7 @@ hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width,
 }
 
 hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
-                        void* scratch, hipStream_t stream)
+                        void* scratch, bool accumulate, hipStream_t stream)
 {
     const uint32_t bw = width / 4, nblocks = bw * (height / 4);
     const uint32_t g = quality_grid(nblocks);
@@ -360,12 +367,13 @@ hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pit
     else
         hipLaunchKernelGGL(k_sqerr<1>, dim3(g), dim3(256), 0, stream, (const uint2*)blocks, (const uint8_t*)src,
                            (uint64_t)pitch, bw, nblocks, magic, part);
-    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out);
+    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out,
+                       accumulate ? 1u : 0u);
     return hipGetLastError();
 }
 
 hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
-                               uint32_t height, void* scratch, hipStream_t stream)
+                               uint32_t height, void* scratch, bool accumulate, hipStream_t stream)
 {
     const uint32_t qw = width / 4, n = qw * height;
     const uint32_t g = quality_grid(n);
@@ -373,7 +381,8 @@ hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, siz
     QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(scratch) + kQualityPartialsOffset);
     hipLaunchKernelGGL(k_sqerr_images, dim3(g), dim3(256), 0, stream, (const uint8_t*)a, (uint64_t)pitch_a,
                        (const uint8_t*)b, (uint64_t)pitch_b, qw, n, div_magic(qw), part);
-    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out);
+    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out,
+                       accumulate ? 1u : 0u);
     return hipGetLastError();
 }
 

@@ -119,10 +119,12 @@ inline size_t quality_scratch_bytes(uint64_t units)
 {
     return kQualityPartialsOffset + (size_t)quality_grid(units) * sizeof(QualitySums);
 }
+// accumulate: add onto the sums already at offset 0 of scratch (later
+// stripes of one image on the same stream) instead of replacing them.
 hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
-                        void* scratch, hipStream_t stream);
+                        void* scratch, bool accumulate, hipStream_t stream);
 hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
-                               uint32_t height, void* scratch, hipStream_t stream);
+                               uint32_t height, void* scratch, bool accumulate, hipStream_t stream);
 hipError_t launch_rgb565_baseline(const void* src, size_t src_pitch, void* dst, size_t dst_pitch, uint32_t width,
                                   uint32_t height, hipStream_t stream, hipEvent_t start = nullptr,
                                   hipEvent_t stop = nullptr);

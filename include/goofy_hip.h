@@ -31,9 +31,9 @@ typedef enum goofy_status {
     GOOFY_ERR_INVALID = -4,        /* null pointer, bad format, stride/pitch too small or misaligned, bad config */
     GOOFY_ERR_OUT_OF_MEMORY = -5,  /* device or pinned allocation failed */
     GOOFY_ERR_NO_DEVICE = -6,      /* no GPU visible */
-    GOOFY_ERR_TOO_LARGE = -7,      /* encode/decode/batch: rows of 4,194,304 pixels or more (larger images of
-                                      narrower rows run as horizontal stripes); quality sums, probe and
-                                      serving: blocks * blocks-per-row >= 2^40 */
+    GOOFY_ERR_TOO_LARGE = -7,      /* rows of 4,194,304 pixels or more (larger images of narrower rows run
+                                      as horizontal stripes); probe and serving: blocks * blocks-per-row
+                                      >= 2^40 */
     GOOFY_ERR_BUSY = -8            /* serving: admission refused (budget or queue full) */
 } goofy_status;
 

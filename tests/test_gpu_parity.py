@@ -177,7 +177,16 @@ def test_images_beyond_one_launch_are_striped(gpu, oracle):
         assert np.array_equal(to_host(got_buf), want)
         dec = dev_buffer(pitch * h, fill=0x3C)
         gpu.decode_device(fmt, dec, got_buf, w, h, pitch)
-        assert np.array_equal(to_host(dec), oracle.decode_image(fmt, want, w, h))
+        dec_want = oracle.decode_image(fmt, want, w, h)
+        assert np.array_equal(to_host(dec), dec_want)
+        # quality sums over the stripes: integer squared errors, and 10^4 x the
+        # luma sum as the exact integer sum of (21 dr + 72 dg + 7 db)^2
+        d = img.reshape(-1, 4).astype(np.int64) - dec_want.reshape(-1, 4)
+        sse = [int(x) for x in (d * d).sum(0)]
+        y100 = int(((21 * d[:, 2] + 72 * d[:, 1] + 7 * d[:, 0]) ** 2).sum())
+        for q in (gpu.quality_device(fmt, got_buf, src, w, h, pitch), gpu.quality_images_device(src, dec, w, h)):
+            assert q["sse_byte"] == sse
+            assert q["mse_y"] == pytest.approx(y100 / 1e4 / (w * h), rel=1e-12)
         got_buf.fill(0xCD)
         small = dev_buffer(64 * 64 * 4, fill=7)
         small_out = dev_buffer(16 * 16 * 8, fill=0xCD)
@@ -189,6 +198,9 @@ def test_images_beyond_one_launch_are_striped(gpu, oracle):
         assert np.array_equal(to_host(small_out),
                               oracle.compress_blocks(fmt, np.full(64 * 64 * 4, 7, np.uint8), 64, 64, 256))
         b.close()
+    floor = dev_buffer(pitch * h, fill=0)
+    gpu.rgb565_baseline_device(floor, src, w, h)
+    assert np.array_equal(to_host(floor), oracle.rgb565_baseline(img, w, h))
 
 
 def test_65536_wide_image_striped_vs_reference(gpu, reference):
