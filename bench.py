@@ -13,6 +13,8 @@ Also reported:
   roofline of the slower kernel, beside a traffic probe (same addresses, no
   encode arithmetic, best of four workgroup sizes) that gives the attainable
   floor;
+* both formats from one read of the 8192^2 image (the dual-format entry
+  point);
 * the 16384^2 stress image with its padded pitch (default and tuned launch
   shapes);
 * the decoders, the RGB565 quality floor and the quality sums on the 8192^2
@@ -137,6 +139,48 @@ def bench_plain(g, hip, W, steps: int, warmup: int, rotate: int, cfg: dict | Non
     stream.destroy()
     stream_b.destroy()
     return im, elapsed, kern_ms, kern_med, parity
+
+
+# ---- both formats in one pass ------------------------------------------------------------------
+
+def bench_dual(g, hip, W, launches: int = 200, rotate: int = 4):
+    """goofy_hip_compress_device_dual on the 8192^2 image: one read of the image,
+    DXT1 and ETC1S written (64 + 16 B per block).  Kernel time from
+    dispatch-recorded events over `launches` launches on one stream, inputs
+    rotated; outputs checked against the reference's digests."""
+    im = W.IMAGE_8192
+    digests = json.loads((ROOT / "tests" / "golden" / "digests.json").read_text())["images"][im.name]
+    stream = hip.Stream()
+    srcs = [hip.DeviceBuffer(im.pitch * im.height) for _ in range(rotate)]
+    for s_ in srcs:
+        g.synth_device(im.kind, im.seed, im.texture_id, im.noise, s_, im.width, im.height, im.pitch, stream)
+    a, b = hip.DeviceBuffer(im.blocks * 8), hip.DeviceBuffer(im.blocks * 8)
+    g.compress_device_dual(a, b, srcs[0], im.width, im.height, im.pitch, stream)
+    stream.synchronize()
+    exact = sha(a.download()) == digests["dxt1_sha256"] and sha(b.download()) == digests["etc1_sha256"]
+    for i in range(200):  # warm-up
+        g.compress_device_dual(a, b, srcs[i % rotate], im.width, im.height, im.pitch, stream)
+    stream.synchronize()
+    # wall clock over back-to-back launches, then kernel time from events
+    t0 = time.perf_counter()
+    for i in range(launches):
+        g.compress_device_dual(a, b, srcs[i % rotate], im.width, im.height, im.pitch, stream)
+    stream.synchronize()
+    wall_ms = (time.perf_counter() - t0) / launches * 1e3
+    ev = [(hip.Event(), hip.Event()) for _ in range(launches)]
+    for i, (e0, e1) in enumerate(ev):
+        g.compress_device_dual(a, b, srcs[i % rotate], im.width, im.height, im.pitch, stream, e0, e1)
+    stream.synchronize()
+    ms = float(np.mean([e0.elapsed_ms(e1) for e0, e1 in ev]))
+    algo = im.blocks * 80
+    res = {"workload": f"{im.name} RGBA8, DXT1 + ETC1S from one read of the image",
+           "kernel_ms_mean": round(ms, 5), "wall_ms_per_launch": round(wall_ms, 5),
+           "mps_both_formats": round(2 * im.pixels / 1e6 / (wall_ms / 1e3), 1),
+           "gbps": round(algo / (ms / 1e3) / 1e9, 1), "algorithmic_bytes_per_launch": algo, "bit_exact": exact}
+    for x in srcs + [a, b]:
+        x.free()
+    stream.destroy()
+    return res
 
 
 # ---- branch-coverage stress image ---------------------------------------------------------------
@@ -462,6 +506,7 @@ def main() -> int:
     ap.add_argument("--skip-dropin", action="store_true")
     ap.add_argument("--skip-stress", action="store_true")
     ap.add_argument("--skip-around", action="store_true")
+    ap.add_argument("--skip-dual", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--profile", type=int, default=0, help="rocprof mode: N launches per kernel, no JSON")
     ap.add_argument("--sweep", action="store_true", help="launch-shape sweep table, no JSON")
@@ -524,6 +569,9 @@ def main() -> int:
                    "image": f"{im.width}x{im.height}", "formats": ["DXT1", "ETC1S"], "parallelism": "single GPU"},
         "roofline": roofline, "formats": formats, "parity": parity,
     }
+    if not args.skip_dual:
+        log("dual-format encode 8192^2")
+        result["dual"] = bench_dual(g, hip, W)
     if not args.skip_stress:
         log("stress 16384^2")
         result["stress"] = bench_stress(g, hip, W)

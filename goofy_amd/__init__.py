@@ -182,6 +182,17 @@ def compress_device(fmt: int, dst, src, width: int, height: int, pitch: int, str
                                                   _stream(stream), ctypes.byref(cfg)))
 
 
+_sig("goofy_hip_compress_device_dual", ctypes.c_int, _vp, _vp, _vp, _uint, _uint, _sz, _vp, _vp, _vp)
+
+
+def compress_device_dual(dst_dxt1, dst_etc1, src, width: int, height: int, pitch: int, stream=None,
+                         start_event=None, stop_event=None) -> None:
+    """Both formats of a device-resident image in one pass (one read of the image)."""
+    _check(_lib.goofy_hip_compress_device_dual(_ptr(dst_dxt1), _ptr(dst_etc1), _ptr(src), width, height, pitch,
+                                               _stream(stream), start_event.handle if start_event else None,
+                                               stop_event.handle if stop_event else None))
+
+
 _sig("goofy_hip_tune_device", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp, ctypes.c_int,
      ctypes.POINTER(LaunchConfig), ctypes.POINTER(ctypes.c_float))
 
@@ -372,5 +383,5 @@ __all__ = [
     "compressDXT1", "compressETC1", "compress", "compress_device", "compress_dxt1_fn", "compress_etc1_fn",
     "synth_host", "synth_device", "last_error", "version", "device_count", "WAVES", "STRESS",
     "decode_device", "quality_device", "compress_device_timed", "quality_images_device", "rgb565_baseline_device",
-    "tune_device",
+    "tune_device", "compress_device_dual",
 ]

@@ -215,6 +215,97 @@ __device__ __forceinline__ uint2 encode_etc1(const uint32_t (&p)[16])
     return make_uint2(w0, w1);
 }
 
+// ---- both formats from one classification (the dual-format kernel) --------
+// The same arithmetic as encode_dxt1 / encode_etc1 above, split into the part
+// both formats share -- statistics and, per row r, the per-byte masks of
+// (y >= mid) and (|y - mid| < qt) -- and the packing of each format.  The
+// single-format encoders keep their own instruction order.
+//
+// What both formats need from a block.
+struct BlockClass {
+    BlockStats s;
+    uint32_t gez[4], near[4];
+};
+
+__device__ __forceinline__ BlockClass classify(const uint32_t (&p)[16])
+{
+    BlockClass c;
+    c.s = block_stats(p);
+    const uint32_t lo = c.s.mid + 1u > c.s.qt ? c.s.mid + 1u - c.s.qt : 0u;          // y >= lo
+    const uint32_t hi = c.s.mid + c.s.qt - 1u < 255u ? c.s.mid + c.s.qt - 1u : 255u;  // y <= hi
+    const uint32_t nmid4 = rep4(~c.s.mid);
+    const uint32_t nlo4 = rep4(~lo);
+    const uint32_t hi4 = rep4(hi);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t y4 = luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]);
+        compare4(y4, nmid4, nlo4, hi4, &c.gez[r], &c.near[r]);
+    }
+    return c;
+}
+
+// DXT1 words from a classified block (as encode_dxt1).
+__device__ __forceinline__ uint2 pack_dxt1(const BlockClass& c)
+{
+    // code byte per pixel: bit0 = y < mid, bit1 = near; row r in word r
+    uint32_t idx = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t code = ((~c.gez[r] >> 7) & 0x01010101u) | ((c.near[r] >> 6) & 0x02020202u);
+        // bytes (c0,c1,c2,c3) with 2-bit codes -> one byte c0|c1<<2|c2<<4|c3<<6
+        uint32_t t = code | (code >> 6);
+        t = t | (t >> 12);
+        idx |= (t & 0xFFu) << (8 * r);
+    }
+    // RGB565 endpoints; green LSB of c0 set so c0 > c1 (4-colour mode)
+    const BlockStats& s = c.s;
+    const uint32_t c0 = 0x20u | (q5(s.mx & 0xFFu) << 11) | (q5((s.mx >> 8) & 0xFFu) << 6) | q5((s.mx >> 16) & 0xFFu);
+    const uint32_t c1 = (q5(s.mn & 0xFFu) << 11) | (q5((s.mn >> 8) & 0xFFu) << 6) | q5((s.mn >> 16) & 0xFFu);
+    return make_uint2(c0 | (c1 << 16), idx);
+}
+
+// ETC1S words from a classified block and its pixels (as encode_etc1).
+__device__ __forceinline__ uint2 pack_etc1(const uint32_t (&p)[16], const BlockClass& c)
+{
+    // Selector planes.  Row r's four MSB flags (bit 8c+7) go to bit 8c+r, so
+    // byte c of G/N holds column c's nibble (rows 0..3).
+    uint32_t G = 0, N = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        G |= (c.gez[r] & 0x80808080u) >> (7 - r);
+        N |= (c.near[r] & 0x80808080u) >> (7 - r);
+    }
+    // Word bit j <-> column {2,3,0,1}[j>>2], row j&3 (goofy_tc.h:1375-1395):
+    // fold nibbles (byte0 = col0|col1<<4, byte2 = col2|col3<<4), then put
+    // G's pair in bits 0..15 and N's in 16..31 and invert.
+    G |= G >> 4;
+    N |= N >> 4;
+    const uint32_t w1 = ~perm(N, G, 0x04060002u);
+
+    // Average colour: rows first, then columns (goofy_tc.h:1402-1414).
+    uint32_t col[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) col[x] = avg4(avg4(p[x], p[4 + x]), avg4(p[8 + x], p[12 + x]));
+    const uint32_t avgc = avg4(avg4(col[0], col[1]), avg4(col[2], col[3]));
+    // Shift its brightness onto mid (goofy_tc.h:1431-1449).
+    const uint32_t ya = luma_b0(avgc);
+    const bool up = ya <= c.s.mid;
+    uint32_t corr = up ? c.s.mid - ya : ya - c.s.mid;
+    corr = corr < 127u ? corr : 127u;
+    uint32_t w0 = 0;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const uint32_t a = (avgc >> (8 * ch)) & 0xFFu;
+        uint32_t b = up ? a + corr : (a > corr ? a - corr : 0u);
+        b = b < 255u ? b : 255u;
+        // q5(b) << 3 == subsat(b,1) & 0xf8
+        w0 |= ((b - (b != 0u ? 1u : 0u)) & 0xF8u) << (8 * ch);
+    }
+    const uint32_t cw = etc1_table(c.s.range);
+    w0 |= ((cw << 5) | (cw << 2) | 3u) << 24;
+    return make_uint2(w0, w1);
+}
+
 template <int FMT>
 __device__ __forceinline__ uint2 encode_block(const uint32_t (&p)[16])
 {

@@ -398,6 +398,40 @@ int goofy_hip_compress_device_timed(int format, void* d_result, const void* d_in
     return compress_device(format, d_result, d_input, width, height, pitch, stream, cfg, start_event, stop_event);
 }
 
+int goofy_hip_compress_device_dual(void* d_dxt1, void* d_etc1, const void* d_input, unsigned int width,
+                                   unsigned int height, size_t pitch, void* stream, void* start_event,
+                                   void* stop_event)
+{
+    clear_error();
+    if (width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
+    if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
+    if (width == 0 || height == 0) return GOOFY_OK;
+    if (!d_dxt1 || !d_etc1 || !d_input) return fail(GOOFY_ERR_INVALID, "null buffer");
+    if (pitch < (size_t)width * 4 || pitch % 16 != 0)
+        return fail(GOOFY_ERR_INVALID, "pitch must be >= 4*width and a multiple of 16");
+    if ((uintptr_t)d_input % 16 != 0 || (uintptr_t)d_dxt1 % 8 != 0 || (uintptr_t)d_etc1 % 8 != 0)
+        return fail(GOOFY_ERR_INVALID, "input must be 16-byte and results 8-byte aligned");
+    const uint32_t bw = width / 4, bh = height / 4;
+    const uint32_t max_rows = max_stripe_brows(bw);
+    if (max_rows == 0) return fail(GOOFY_ERR_TOO_LARGE, "image too wide (4,194,304 pixels or more per row)");
+    DeviceInfo di;
+    int rc = device_info(&di);
+    if (rc != GOOFY_OK) return rc;
+    for (uint32_t y0 = 0; y0 < bh;) {  // stripes for images beyond one launch's index range
+        const uint32_t rows = std::min(max_rows, bh - y0);
+        const uint32_t nb = rows * bw;
+        const size_t out_off = (size_t)y0 * bw * 8;
+        hipError_t e = launch_dual((uint32_t)auto_cfg(nb, di.num_cus).wg_waves,
+                                   static_cast<const uint8_t*>(d_input) + (size_t)y0 * 4 * pitch, pitch,
+                                   static_cast<uint8_t*>(d_dxt1) + out_off, static_cast<uint8_t*>(d_etc1) + out_off,
+                                   bw, nb, (hipStream_t)stream, y0 == 0 ? (hipEvent_t)start_event : nullptr,
+                                   y0 + rows == bh ? (hipEvent_t)stop_event : nullptr);
+        if (e != hipSuccess) return fail_hip(e, "dual encode kernel launch");
+        y0 += rows;
+    }
+    return GOOFY_OK;
+}
+
 int goofy_hip_tune_device(int format, void* d_result, const void* d_input, unsigned int width, unsigned int height,
                           size_t pitch, void* stream, int reps, goofy_launch_config* best, float* best_ms)
 {

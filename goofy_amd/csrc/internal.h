@@ -91,6 +91,8 @@ inline uint32_t max_stripe_brows(uint64_t bw)
 // Kernel launchers (kernels.hip).
 hipError_t launch_plain(int fmt, const LaunchCfg& cfg, const void* src, size_t pitch, void* dst, uint32_t bw,
                         uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+hipError_t launch_dual(uint32_t wg_waves, const void* src, size_t pitch, void* dst_dxt1, void* dst_etc1, uint32_t bw,
+                       uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_batch(const LaunchCfg& cfg, const TexDesc* d_descs, uint32_t ntex, uint32_t ntiles, int num_cus,
                         hipStream_t stream);
 hipError_t launch_batch_inline(const LaunchCfg& cfg, const InlineWorklist& wl, uint32_t ntex, uint32_t ntiles,

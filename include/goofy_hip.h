@@ -98,6 +98,18 @@ GOOFY_HIP_API int goofy_hip_compress_device_timed(int format, void* d_result, co
                                                   unsigned int width, unsigned int height, size_t pitch, void* stream,
                                                   const goofy_launch_config* cfg, void* start_event, void* stop_event);
 
+/* One device-resident image into both formats in one pass: each block is
+ * read and classified once and packed as DXT1 into d_dxt1 and as ETC1S into
+ * d_etc1 (each (width/4)*(height/4)*8 bytes, 8-byte aligned), bit-identical to
+ * the two single-format encodes.  Reads the image once: 80 bytes of traffic
+ * per block instead of 2 x 72 for two launches.  No counterpart in the
+ * reference, whose API encodes one format per call; for pipelines that ship
+ * both.  Same constraints as goofy_hip_compress_device; events (either may be
+ * NULL) as in goofy_hip_compress_device_timed. */
+GOOFY_HIP_API int goofy_hip_compress_device_dual(void* d_dxt1, void* d_etc1, const void* d_input, unsigned int width,
+                                                 unsigned int height, size_t pitch, void* stream, void* start_event,
+                                                 void* stop_event);
+
 /* Launch-shape autotuner for one image shape.  Times the encode kernel under
  * each candidate shape (1, 2 or 4 blocks per lane; workgroups of 1, 2, 4 or 8
  * waves; both tile orders; streaming loads and stores) on the caller's

@@ -127,6 +127,18 @@ def test_floor_and_image_quality_validation_without_device():
     assert e.value.status == goofy_amd.ERR_WIDTH
 
 
+def test_dual_validation_without_device():
+    for args, status in [((8, 8, 16, 6, 4, 32), goofy_amd.ERR_WIDTH),
+                         ((8, 8, 16, 8, 6, 32), goofy_amd.ERR_HEIGHT),
+                         ((8, 8, 16, 8, 4, 24), goofy_amd.ERR_INVALID),     # pitch % 16
+                         ((8, 12, 16, 8, 4, 32), goofy_amd.ERR_INVALID),    # misaligned ETC1S output
+                         ((8, 0, 16, 8, 4, 32), goofy_amd.ERR_INVALID),     # null output
+                         ((8, 8, 16, 4194304, 4, 4194304 * 4), goofy_amd.ERR_TOO_LARGE)]:
+        with pytest.raises(goofy_amd.GoofyError) as e:
+            goofy_amd.compress_device_dual(*args)
+        assert e.value.status == status, args
+
+
 def test_too_wide_images_refused_without_device():
     # 4,194,304 pixels per row (2^20 blocks) is beyond one launch's index range
     # even for a single block row; narrower images of any height are striped

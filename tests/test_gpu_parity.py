@@ -226,6 +226,56 @@ def test_65536_wide_image_striped_vs_reference(gpu, reference):
     del img
 
 
+# ---- both formats in one pass ----------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["waves-8192x8192", "stress-16384x16384-pitch65792"])
+def test_dual_golden_digests(gpu, digests, name):
+    im = {W.IMAGE_8192.name: W.IMAGE_8192, W.STRESS_16384.name: W.STRESS_16384}[name]
+    d = digests["images"][name]
+    src = dev_buffer(im.pitch * im.height, fill=0)
+    gpu.synth_device(im.kind, im.seed, im.texture_id, im.noise, src, im.width, im.height, im.pitch)
+    a = dev_buffer(im.blocks * 8, fill=0xCD)
+    b = dev_buffer(im.blocks * 8, fill=0xCD)
+    say(f"dual {name}")
+    gpu.compress_device_dual(a, b, src, im.width, im.height, im.pitch)
+    assert sha(to_host(a)) == d["dxt1_sha256"]
+    assert sha(to_host(b)) == d["etc1_sha256"]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_dual_shapes_vs_oracle(gpu, oracle, seed):
+    rng = np.random.default_rng(100 + seed)
+    w = int(rng.integers(1, 400)) * 4
+    h = int(rng.integers(1, 80)) * 4
+    pitch = w * 4 + int(rng.integers(0, 4)) * 16
+    img = rng.integers(0, 256, pitch * h, dtype=np.uint8)
+    if seed % 2:
+        img = np.clip(img // 8 + rng.integers(0, 220), 0, 255).astype(np.uint8)
+    src = upload(img)
+    n = (w // 4) * (h // 4) * 8
+    a, b = dev_buffer(n, fill=0xCD), dev_buffer(n, fill=0xCD)
+    say(f"dual shape seed={seed} {w}x{h} pitch={pitch}")
+    gpu.compress_device_dual(a, b, src, w, h, pitch)
+    assert np.array_equal(to_host(a), oracle.compress_blocks(W.DXT1, img, w, h, pitch))
+    assert np.array_equal(to_host(b), oracle.compress_blocks(W.ETC1, img, w, h, pitch))
+
+
+def test_dual_striped_and_random_blocks(gpu, reference):
+    """The dual kernel on 2^19-block rows (stripes of 3 + 1 block rows) of
+    threshold-kind random blocks, against the reference encoder."""
+    from blockgen import block_image
+    bw, bh = 1 << 19, 4
+    w, h = 4 * bw, 4 * bh
+    img = block_image("threshold", 77, bw, bh)
+    src = upload(img)
+    n = bw * bh * 8
+    a, b = dev_buffer(n, fill=0xCD), dev_buffer(n, fill=0xCD)
+    say(f"dual striped {w}x{h}")
+    gpu.compress_device_dual(a, b, src, w, h, w * 4)
+    assert np.array_equal(to_host(a), reference.compress_mt(W.DXT1, img, w, h, w * 4, 16))
+    assert np.array_equal(to_host(b), reference.compress_mt(W.ETC1, img, w, h, w * 4, 16))
+
+
 @pytest.mark.parametrize("bpl", [1, 2, 4])
 @pytest.mark.parametrize("wg", [1, 2, 4, 8])
 @pytest.mark.parametrize("nt", [0, 1])
