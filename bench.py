@@ -430,9 +430,10 @@ def load_traffic():
 def profile_mode(g, hip, W, launches: int):
     """Short run for rocprofv3, all on one stream: each plain kernel `launches`
     times (default launch shape), the traffic probe `launches` times per
-    workgroup size (1, 2, 4, 8 waves), and a device-to-device copy of the
-    256 MiB image (the runtime's copy kernel) used to calibrate the HBM byte
-    counters.  Inputs rotate over four 256 MiB buffers."""
+    workgroup size (1, 2, 4, 8 waves), the dual-format kernel `launches` times,
+    and a device-to-device copy of the 256 MiB image (the runtime's copy
+    kernel) used to calibrate the HBM byte counters.  Inputs rotate over four
+    256 MiB buffers."""
     im = W.IMAGE_8192
     nbytes = im.pitch * im.height
     srcs = [hip.DeviceBuffer(nbytes) for _ in range(4)]
@@ -449,6 +450,9 @@ def profile_mode(g, hip, W, launches: int):
     for wg in (1, 2, 4, 8):
         for i in range(launches):
             g.traffic_probe(out, srcs[i % 4], im.width, im.height, im.pitch, wg_waves=wg)
+    out2 = hip.DeviceBuffer(im.blocks * 8)
+    for i in range(launches):
+        g.compress_device_dual(out, out2, srcs[i % 4], im.width, im.height, im.pitch)
     hip.synchronize()
     log(f"profile mode done: {launches} launches per kernel")
 
