@@ -362,6 +362,16 @@ def bench_dropin(g, W, reps: int = 5):
                 best = min(best, time.perf_counter() - t0)
             res[f"{im.name}_{key}"] = {"ms": round(best * 1e3, 3), "mps": round(im.pixels / 1e6 / best, 1),
                                         "pcie_gbps": round((im.pitch * im.height + im.blocks * 8) / best / 1e9, 2)}
+        # both formats from one transfer of the image (goofy_hip_compress_dual)
+        pair = (out, np.empty(im.blocks * 8, np.uint8))
+        g.compress_dual(img, im.width, im.height, im.pitch, out=pair)
+        best = 1e9
+        for _ in range(reps if im is W.IMAGE_8192 else 4 * reps):
+            t0 = time.perf_counter()
+            g.compress_dual(img, im.width, im.height, im.pitch, out=pair)
+            best = min(best, time.perf_counter() - t0)
+        res[f"{im.name}_dual"] = {"ms": round(best * 1e3, 3), "mps": round(im.pixels / 1e6 / best, 1),
+                                  "pcie_gbps": round((im.pitch * im.height + im.blocks * 16) / best / 1e9, 2)}
     return res
 
 

@@ -169,6 +169,26 @@ def compress(fmt: int, image: np.ndarray, width: int, height: int, stride: int |
     return out[: (width // 4) * (height // 4) * 8]
 
 
+_sig("goofy_hip_compress_dual", ctypes.c_int, _u8p, _u8p, _u8p, _uint, _uint, _uint)
+
+
+def compress_dual(image: np.ndarray, width: int, height: int, stride: int | None = None, out=None) -> tuple:
+    """Both formats of a host RGBA8 image from one transfer and one pass: (dxt1, etc1s) block bytes.
+    `out`: an optional pair of uint8 arrays of at least (w/4)*(h/4)*8 bytes each to write into."""
+    stride = width * 4 if stride is None else stride
+    n = (width // 4) * (height // 4) * 8
+    if out is None:
+        a = np.empty(max(1, n), dtype=np.uint8)
+        b = np.empty(max(1, n), dtype=np.uint8)
+    else:
+        a, b = out
+        if a.dtype != np.uint8 or b.dtype != np.uint8 or a.size < n or b.size < n or not (
+                a.flags.c_contiguous and b.flags.c_contiguous):
+            raise ValueError(f"out needs two contiguous uint8 arrays of >= {n} bytes")
+    _check(_lib.goofy_hip_compress_dual(_u8(a), _u8(b), _u8(image), width, height, stride))
+    return a[:n], b[:n]
+
+
 # ---- device memory ----------------------------------------------------------------
 
 def compress_device(fmt: int, dst, src, width: int, height: int, pitch: int, stream=None,

@@ -212,16 +212,22 @@ static void copy_rows(uint8_t* dst, const uint8_t* src, size_t stride, size_t ro
     });
 }
 
+// fmt: GOOFY_FORMAT_DXT1 or GOOFY_FORMAT_ETC1 into `result`, or kHostDual:
+// DXT1 into `result` and ETC1S into `result2` from one pass over the image.
+constexpr int kHostDual = 1 << 16;
+
 static int compress_host(int fmt, unsigned char* result, const unsigned char* input, unsigned width,
-                         unsigned height, unsigned stride)
+                         unsigned height, unsigned stride, unsigned char* result2 = nullptr)
 {
     clear_error();
-    if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
+    const bool dual = fmt == kHostDual;
+    if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1 && !dual)
+        return fail(GOOFY_ERR_INVALID, "unknown format");
     // Same checks, same order as the reference (GoofyTC/goofy_tc.h:1500-1508).
     if (width % 16 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 16");
     if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
     if (width == 0 || height == 0) return GOOFY_OK;
-    if (!result || !input) return fail(GOOFY_ERR_INVALID, "null buffer");
+    if (!result || !input || (dual && !result2)) return fail(GOOFY_ERR_INVALID, "null buffer");
     if ((size_t)stride < (size_t)width * 4) return fail(GOOFY_ERR_INVALID, "stride smaller than 4*width");
     const uint32_t bw = width / 4, bh = height / 4;
     const uint32_t max_rows = max_stripe_brows(bw);  // per chunk: chunks are launched one by one
@@ -239,7 +245,9 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
     const size_t row_bytes = (size_t)width * 4;
     const unsigned chunk_brows = (unsigned)std::max<size_t>(1, (size_t(8) << 20) / (row_bytes * 4));  // ~8 MiB
     const unsigned cb = std::min(std::min(chunk_brows, bh), max_rows);
-    if ((rc = ensure_ctx(ctx, row_bytes * 4 * cb, (size_t)bw * cb * 8)) != GOOFY_OK) return rc;
+    // dual: a chunk's staging output holds its DXT1 blocks, then its ETC1S blocks
+    const size_t outs = dual ? 2 : 1;
+    if ((rc = ensure_ctx(ctx, row_bytes * 4 * cb, outs * bw * cb * 8)) != GOOFY_OK) return rc;
     const unsigned nchunks = (bh + cb - 1) / cb;
     const LaunchCfg cfg = auto_cfg((uint64_t)bw * cb, di.num_cus);
     hipError_t e = hipSuccess;
@@ -249,7 +257,9 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
         const unsigned rows = std::min(cb, bh - k * cb);
         hipError_t r = hipEventSynchronize(ctx->out_ready[s]);
         if (r != hipSuccess) return r;
-        std::memcpy(result + (size_t)k * cb * bw * 8, ctx->h_out[s], (size_t)rows * bw * 8);
+        const size_t n = (size_t)rows * bw * 8;
+        std::memcpy(result + (size_t)k * cb * bw * 8, ctx->h_out[s], n);
+        if (dual) std::memcpy(result2 + (size_t)k * cb * bw * 8, static_cast<uint8_t*>(ctx->h_out[s]) + n, n);
         return hipSuccess;
     };
 
@@ -263,11 +273,13 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
                                 ctx->stream)) != hipSuccess)
             break;
         if ((e = hipEventRecord(ctx->h2d_done[s], ctx->stream)) != hipSuccess) break;
-        if ((e = launch_plain(fmt, cfg, ctx->d_in[s], row_bytes, ctx->d_out[s], bw, bw * rows, ctx->stream)) !=
+        const size_t n = (size_t)bw * rows * 8;
+        e = dual ? launch_dual((uint32_t)cfg.wg_waves, ctx->d_in[s], row_bytes, ctx->d_out[s],
+                               static_cast<uint8_t*>(ctx->d_out[s]) + n, bw, bw * rows, ctx->stream)
+                 : launch_plain(fmt, cfg, ctx->d_in[s], row_bytes, ctx->d_out[s], bw, bw * rows, ctx->stream);
+        if (e != hipSuccess) break;
+        if ((e = hipMemcpyAsync(ctx->h_out[s], ctx->d_out[s], outs * n, hipMemcpyDeviceToHost, ctx->stream)) !=
             hipSuccess)
-            break;
-        if ((e = hipMemcpyAsync(ctx->h_out[s], ctx->d_out[s], (size_t)bw * rows * 8, hipMemcpyDeviceToHost,
-                                ctx->stream)) != hipSuccess)
             break;
         if ((e = hipEventRecord(ctx->out_ready[s], ctx->stream)) != hipSuccess) break;
         if (k >= 1) e = finish_chunk(k - 1);
@@ -396,6 +408,12 @@ int goofy_hip_compress_device_timed(int format, void* d_result, const void* d_in
                                     void* start_event, void* stop_event)
 {
     return compress_device(format, d_result, d_input, width, height, pitch, stream, cfg, start_event, stop_event);
+}
+
+int goofy_hip_compress_dual(unsigned char* result_dxt1, unsigned char* result_etc1, const unsigned char* input,
+                            unsigned int width, unsigned int height, unsigned int stride)
+{
+    return compress_host(kHostDual, result_dxt1, input, width, height, stride, result_etc1);
 }
 
 int goofy_hip_compress_device_dual(void* d_dxt1, void* d_etc1, const void* d_input, unsigned int width,

@@ -61,6 +61,15 @@ GOOFY_HIP_API int goofy_hip_compress_dxt1(unsigned char* result, const unsigned 
 GOOFY_HIP_API int goofy_hip_compress_etc1(unsigned char* result, const unsigned char* input, unsigned int width,
                                           unsigned int height, unsigned int stride);
 
+/* Both formats of a host image from one pass: DXT1 blocks into result_dxt1
+ * and ETC1S blocks into result_etc1 (each (width/4)*(height/4)*8 bytes),
+ * bit-identical to goofy_hip_compress_dxt1 and _etc1.  The image crosses PCIe
+ * once instead of twice.  Same checks, codes and synchronous behaviour as the
+ * drop-in.  No counterpart in the reference (one format per call there). */
+GOOFY_HIP_API int goofy_hip_compress_dual(unsigned char* result_dxt1, unsigned char* result_etc1,
+                                          const unsigned char* input, unsigned int width, unsigned int height,
+                                          unsigned int stride);
+
 /* ---- device memory, asynchronous ------------------------------------------
  * Encodes a device-resident RGBA8 image into device-resident blocks on
  * `stream` (a hipStream_t, NULL = default stream) and returns after the

@@ -71,6 +71,17 @@ def test_dropin_return_codes_without_device(fn):
     assert fn(out, img, 64, 0, 256) == 0
 
 
+def test_dual_dropin_return_codes_without_device():
+    out_a, out_b = np.zeros(64, np.uint8), np.zeros(64, np.uint8)
+    img = np.zeros(64 * 64 * 4, np.uint8)
+    f = goofy_amd._lib.goofy_hip_compress_dual
+    u8 = goofy_amd._u8
+    assert f(u8(out_a), u8(out_b), u8(img), 60, 64, 240) == -1
+    assert f(u8(out_a), u8(out_b), u8(img), 64, 62, 256) == -2
+    assert f(u8(out_a), u8(out_b), u8(img), 0, 64, 0) == 0
+    assert f(u8(out_a), u8(None), u8(img), 64, 64, 256) == goofy_amd.ERR_INVALID
+
+
 def test_compress_func_type_matches_reference_plugin():
     # CompressFunc_t (Src/main.cpp:644): int(uchar* dst, const uchar* src, uint w, uint h, uint stride)
     out = np.zeros(64, np.uint8)

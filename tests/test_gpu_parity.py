@@ -483,3 +483,33 @@ def test_random_blocks_vs_reference(gpu, reference, kind):
         want = reference.compress_mt(fmt, img, w, h, w * 4, 16)
         bad = np.nonzero((got.reshape(-1, 8) != want.reshape(-1, 8)).any(1))[0]
         assert bad.size == 0, f"{bad.size} blocks differ, first {bad[:8]}"
+
+
+@pytest.mark.parametrize("w,h,extra", [(16, 4, 0), (528, 36, 200), (1024, 4100, 12), (4096, 1028, 0)])
+def test_dropin_dual_matches_single_format(gpu, oracle, w, h, extra):
+    """goofy_hip_compress_dual against the two single-format drop-ins; the
+    last two shapes span several ~8 MiB staging chunks, the last chunk short."""
+    rng = np.random.default_rng(w + 3 * h)
+    stride = w * 4 + extra
+    img = rng.integers(0, 256, stride * h, dtype=np.uint8)
+    img[::7] = img[::7] // 8  # some narrow-range blocks too
+    say(f"drop-in dual {w}x{h} stride={stride}")
+    a, b = gpu.compress_dual(img, w, h, stride)
+    n = (w // 4) * (h // 4) * 8
+    for fmt, got, fn in ((W.DXT1, a, gpu.compressDXT1), (W.ETC1, b, gpu.compressETC1)):
+        one = np.full(max(1, n), 0xCD, np.uint8)
+        assert fn(one, img, w, h, stride) == 0
+        assert np.array_equal(got, one[:n])
+        if n <= 1 << 16:
+            rc, want = oracle.compress(fmt, img, w, h, stride)
+            assert rc == 0 and np.array_equal(got, want)
+
+
+def test_dropin_dual_return_codes(gpu):
+    img = np.zeros(64 * 64 * 4, np.uint8)
+    out_a, out_b = np.zeros(4096, np.uint8), np.zeros(4096, np.uint8)
+    f = gpu._lib.goofy_hip_compress_dual
+    assert f(gpu._u8(out_a), gpu._u8(out_b), gpu._u8(img), 60, 64, 240) == -1
+    assert f(gpu._u8(out_a), gpu._u8(out_b), gpu._u8(img), 64, 62, 256) == -2
+    assert f(gpu._u8(out_a), gpu._u8(out_b), gpu._u8(img), 0, 8, 0) == 0
+    assert f(gpu._u8(out_a), gpu._u8(None), gpu._u8(img), 64, 64, 256) == gpu.ERR_INVALID
