@@ -15,6 +15,7 @@ Also reported:
   floor;
 * both formats from one read of the 8192^2 image (the dual-format entry
   point);
+* DXT5 (BC3) and ETC2 RGBA8 on the 8192^2 image with an alpha channel;
 * the 16384^2 stress image with its padded pitch (default and tuned launch
   shapes);
 * the decoders, the RGB565 quality floor and the quality sums on the 8192^2
@@ -178,6 +179,44 @@ def bench_dual(g, hip, W, launches: int = 200, rotate: int = 4):
            "mps_both_formats": round(2 * im.pixels / 1e6 / (wall_ms / 1e3), 1),
            "gbps": round(algo / (ms / 1e3) / 1e9, 1), "algorithmic_bytes_per_launch": algo, "bit_exact": exact}
     for x in srcs + [a, b]:
+        x.free()
+    stream.destroy()
+    return res
+
+
+# ---- formats with alpha ---------------------------------------------------------------------------
+
+def bench_alpha(g, hip, W, launches: int = 200, rotate: int = 4):
+    """DXT5 (BC3) and ETC2 RGBA8 on the 8192^2 image with an alpha channel:
+    64 B read + 16 B written per block.  Kernel time from dispatch-recorded
+    events over `launches` launches per format on one stream, inputs rotated;
+    outputs checked against the golden digests."""
+    im = W.ALPHA_8192
+    digests = json.loads((ROOT / "tests" / "golden" / "digests.json").read_text())["alpha_images"][im.name]
+    stream = hip.Stream()
+    srcs = [hip.DeviceBuffer(im.pitch * im.height) for _ in range(rotate)]
+    for s_ in srcs:
+        g.synth_device(im.kind, im.seed, im.texture_id, im.noise, s_, im.width, im.height, im.pitch, stream)
+    out = hip.DeviceBuffer(im.blocks * 16)
+    algo = im.blocks * 80
+    res = {"workload": f"{im.name} RGBA8 (alpha: opaque, cutout and smooth 64x64 regions)",
+           "algorithmic_bytes_per_launch": algo}
+    for fmt, key in ((W.DXT5, "dxt5"), (W.ETC2, "etc2")):
+        g.compress_device(fmt, out, srcs[0], im.width, im.height, im.pitch, stream)
+        stream.synchronize()
+        exact = sha(out.download()) == digests[f"{key}_sha256"]
+        for i in range(200):  # warm-up
+            g.compress_device(fmt, out, srcs[i % rotate], im.width, im.height, im.pitch, stream)
+        ev = [(hip.Event(), hip.Event()) for _ in range(launches)]
+        for i, (e0, e1) in enumerate(ev):
+            g.compress_device_timed(fmt, out, srcs[i % rotate], im.width, im.height, im.pitch, e0, e1, stream)
+        stream.synchronize()
+        t = np.array([e0.elapsed_ms(e1) for e0, e1 in ev])
+        ms = float(t.mean())
+        res[key] = {"kernel_ms_mean": round(ms, 5), "kernel_ms_median": round(float(np.median(t)), 5),
+                    "kernel_mps": round(im.pixels / 1e6 / (ms / 1e3), 1),
+                    "gbps": round(algo / (ms / 1e3) / 1e9, 1), "bit_exact": exact}
+    for x in srcs + [out]:
         x.free()
     stream.destroy()
     return res
@@ -521,6 +560,7 @@ def main() -> int:
     ap.add_argument("--skip-stress", action="store_true")
     ap.add_argument("--skip-around", action="store_true")
     ap.add_argument("--skip-dual", action="store_true")
+    ap.add_argument("--skip-alpha", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--profile", type=int, default=0, help="rocprof mode: N launches per kernel, no JSON")
     ap.add_argument("--sweep", action="store_true", help="launch-shape sweep table, no JSON")
@@ -586,6 +626,9 @@ def main() -> int:
     if not args.skip_dual:
         log("dual-format encode 8192^2")
         result["dual"] = bench_dual(g, hip, W)
+    if not args.skip_alpha:
+        log("DXT5 / ETC2 RGBA8 8192^2")
+        result["alpha"] = bench_alpha(g, hip, W)
     if not args.skip_stress:
         log("stress 16384^2")
         result["stress"] = bench_stress(g, hip, W)

@@ -24,7 +24,14 @@ _LIB_PATH = Path(__file__).resolve().parent / "libgoofy_hip.so"
 
 DXT1 = 0
 ETC1 = 1
-FORMATS = {"dxt1": DXT1, "etc1": ETC1, "etc1s": ETC1}
+DXT5 = 2   # BC3: alpha block + the DXT1 block (16 bytes)
+ETC2 = 3   # ETC2 RGBA8: EAC alpha block + the ETC1S block (16 bytes)
+FORMATS = {"dxt1": DXT1, "etc1": ETC1, "etc1s": ETC1, "dxt5": DXT5, "etc2": ETC2}
+
+
+def block_bytes(fmt: int) -> int:
+    """Bytes per 4x4 block: 8 for DXT1 / ETC1S, 16 for DXT5 / ETC2 RGBA8."""
+    return 16 if fmt in (DXT5, ETC2) else 8
 
 OK = 0
 ERR_WIDTH = -1
@@ -83,6 +90,9 @@ _sig("goofy_hip_device_count", ctypes.c_int)
 _sig("goofy_hip_compress", ctypes.c_int, ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
 _sig("goofy_hip_compress_dxt1", ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
 _sig("goofy_hip_compress_etc1", ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
+_sig("goofy_hip_compress_dxt5", ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
+_sig("goofy_hip_compress_etc2", ctypes.c_int, _u8p, _u8p, _uint, _uint, _uint)
+_sig("goofy_format_block_bytes", _uint, ctypes.c_int)
 _sig("goofy_hip_compress_device", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp)
 _sig("goofy_hip_compress_device_cfg", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp,
      ctypes.POINTER(LaunchConfig))
@@ -107,6 +117,8 @@ _sig("goofy_synth_fill_device", ctypes.c_int, ctypes.c_int, ctypes.c_uint64, cty
 # The reference-signature entry points, usable wherever a CompressFunc_t is.
 compress_dxt1_fn = CompressFunc(("goofy_hip_compress_dxt1", _lib))
 compress_etc1_fn = CompressFunc(("goofy_hip_compress_etc1", _lib))
+compress_dxt5_fn = CompressFunc(("goofy_hip_compress_dxt5", _lib))
+compress_etc2_fn = CompressFunc(("goofy_hip_compress_etc2", _lib))
 
 
 def last_error() -> str:
@@ -161,12 +173,23 @@ def compressETC1(result: np.ndarray, input: np.ndarray, width: int, height: int,
     return int(_lib.goofy_hip_compress_etc1(_u8(result), _u8(input), width, height, stride))
 
 
+def compressDXT5(result: np.ndarray, input: np.ndarray, width: int, height: int, stride: int) -> int:
+    """DXT5 (BC3) with the drop-in's contract: 16 bytes per block, codes 0 / -1 / -2."""
+    return int(_lib.goofy_hip_compress_dxt5(_u8(result), _u8(input), width, height, stride))
+
+
+def compressETC2(result: np.ndarray, input: np.ndarray, width: int, height: int, stride: int) -> int:
+    """ETC2 RGBA8 (EAC alpha) with the drop-in's contract: 16 bytes per block, codes 0 / -1 / -2."""
+    return int(_lib.goofy_hip_compress_etc2(_u8(result), _u8(input), width, height, stride))
+
+
 def compress(fmt: int, image: np.ndarray, width: int, height: int, stride: int | None = None) -> np.ndarray:
-    """Encode a host RGBA8 image; returns the (w/4)*(h/4)*8 block bytes.  Raises on error."""
+    """Encode a host RGBA8 image; returns the (w/4)*(h/4) blocks' bytes.  Raises on error."""
     stride = width * 4 if stride is None else stride
-    out = np.empty(max(1, (width // 4) * (height // 4) * 8), dtype=np.uint8)
+    n = (width // 4) * (height // 4) * block_bytes(fmt)
+    out = np.empty(max(1, n), dtype=np.uint8)
     _check(_lib.goofy_hip_compress(fmt, _u8(out), _u8(image), width, height, stride))
-    return out[: (width // 4) * (height // 4) * 8]
+    return out[:n]
 
 
 _sig("goofy_hip_compress_dual", ctypes.c_int, _u8p, _u8p, _u8p, _uint, _uint, _uint)
@@ -399,7 +422,8 @@ def synth_device(kind: int, seed: int, texture_id: int, noise: int, dst, width: 
 
 
 __all__ = [
-    "DXT1", "ETC1", "FORMATS", "GoofyError", "LaunchConfig", "TextureDesc", "CompressFunc", "Batch",
+    "DXT1", "ETC1", "DXT5", "ETC2", "FORMATS", "block_bytes", "compressDXT5", "compressETC2",
+    "compress_dxt5_fn", "compress_etc2_fn", "GoofyError", "LaunchConfig", "TextureDesc", "CompressFunc", "Batch",
     "compressDXT1", "compressETC1", "compress", "compress_device", "compress_dxt1_fn", "compress_etc1_fn",
     "synth_host", "synth_device", "last_error", "version", "device_count", "WAVES", "STRESS",
     "decode_device", "quality_device", "compress_device_timed", "quality_images_device", "rgb565_baseline_device",

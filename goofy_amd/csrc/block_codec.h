@@ -25,7 +25,9 @@
 
 namespace goofy_dev {
 
-enum : int { kDXT1 = 0, kETC1 = 1 };
+// DXT5 (BC3) and ETC2 RGBA8 are 16-byte blocks: an alpha block, then the
+// DXT1 / ETC1S block of the same pixels.
+enum : int { kDXT1 = 0, kETC1 = 1, kDXT5 = 2, kETC2 = 3 };
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
@@ -311,6 +313,140 @@ __device__ __forceinline__ uint2 encode_block(const uint32_t (&p)[16])
 {
     if constexpr (FMT == kDXT1) return encode_dxt1(p);
     else return encode_etc1(p);
+}
+
+// ---- alpha blocks (DXT5 / ETC2 RGBA8) ---------------------------------------
+// The reference has no alpha encoder; its README names one that works on
+// alpha directly instead of brightness as the next step.  This one does that
+// with the same byte arithmetic as the colour blocks:
+//   * min/max alpha of the block;
+//   * the 8 palette levels in ascending order, exactly as a decoder forms
+//     them (Src/decoder.cpp:25-42 EAC tables, :874-896 BC3 palette);
+//   * per pixel, its rank among the levels = the number of midpoints
+//     ceil((lv[k-1] + lv[k]) / 2) it reaches, so every pixel takes the level
+//     nearest to it and a pixel exactly between two takes the higher one;
+//   * the rank's selector, packed in the format's bit order.
+// BC3: a0 = max, a1 = min (8-level mode when they differ), level k =
+// min + floor(k*(max-min)/7).  EAC: table 9 ({-10,-8,-5,-2,1,4,7,9} sorted),
+// multiplier m = max(1, ceil((max-min)/19)) so that 19m covers the range,
+// base min + 10m (lowest level on the minimum) or, when that passes 255,
+// max - 9m (highest level on the maximum).
+
+struct AlphaRows {
+    uint32_t w[4];  // word r: byte c = alpha of pixel (row r, column c)
+    uint32_t mn, mx;
+};
+
+__device__ __forceinline__ AlphaRows alpha_rows(const uint32_t (&p)[16])
+{
+    AlphaRows a;
+    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        // [A0 A1 0 0] | [0 0 A2 A3]
+        a.w[r] = perm(p[4 * r + 1], p[4 * r], 0x0c0c0703u) | perm(p[4 * r + 3], p[4 * r + 2], 0x07030c0cu);
+        const uint32_t e = a.w[r] & 0x00FF00FFu, o = (a.w[r] >> 8) & 0x00FF00FFu;
+        mn = pk_min16(mn, pk_min16(e, o));
+        mx = pk_max16(mx, pk_max16(e, o));
+    }
+    a.mn = pk_min16(mn, mn >> 16) & 0xFFu;
+    a.mx = pk_max16(mx, mx >> 16) & 0xFFu;
+    return a;
+}
+
+// Per byte of w: the number (0..7) of thresholds it reaches.  nthr4[k-1] is
+// ~thr_k replicated in all four bytes; avg(~t, v) >= 128 <=> v >= t.
+__device__ __forceinline__ uint32_t rank4(uint32_t w, const uint32_t (&nthr4)[7])
+{
+    uint32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) t += (avg4(nthr4[k], w) >> 7) & 0x01010101u;
+    return t;
+}
+
+// Thresholds between consecutive ascending levels, inverted and replicated.
+__device__ __forceinline__ void midpoints(const uint32_t (&lv)[8], uint32_t (&nthr4)[7])
+{
+#pragma unroll
+    for (int k = 1; k < 8; ++k) nthr4[k - 1] = rep4(255u - ((lv[k - 1] + lv[k] + 1u) >> 1));
+}
+
+// BC3 alpha block as two little-endian words.
+__device__ __forceinline__ uint2 encode_bc3_alpha(const uint32_t (&p)[16])
+{
+    const AlphaRows a = alpha_rows(p);
+    // level k = mn + floor(k*range/7); (x * 9363) >> 16 == x / 7 for x <= 1785
+    const uint32_t r9 = (a.mx - a.mn) * 9363u;
+    uint32_t lv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lv[k] = a.mn + (((uint32_t)k * r9) >> 16);
+    uint32_t nthr4[7];
+    midpoints(lv, nthr4);
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t t = rank4(a.w[r], nthr4);
+        // rank -> index: 0 -> 1 (a1), 7 -> 0 (a0), k -> 8 - k in between
+        const uint32_t u = (0x08080808u - t) & 0x07070707u;
+        const uint32_t idx = u ^ (~((u >> 1) | (u >> 2)) & 0x01010101u);
+        // four 3-bit indices (bytes) -> 12 bits, pixel c at bits 3c
+        const uint32_t y = idx | (idx >> 5);
+        const uint32_t v = (y & 0x3Fu) | ((y >> 10) & 0xFC0u);
+        // row r occupies bits 12r..12r+11 of the 48 index bits
+        if (r == 0) lo = v;
+        else if (r == 1) lo |= v << 12;
+        else if (r == 2) { lo |= v << 24; hi = v >> 8; }
+        else hi |= v << 4;
+    }
+    return make_uint2(a.mx | (a.mn << 8) | (lo << 16), (lo >> 16) | (hi << 16));
+}
+
+// EAC (ETC2 RGBA8 alpha) block as two little-endian words.
+__device__ __forceinline__ uint2 encode_eac_alpha(const uint32_t (&p)[16])
+{
+    const AlphaRows a = alpha_rows(p);
+    const uint32_t range = a.mx - a.mn;
+    uint32_t m = ((range + 18u) * 3450u) >> 16;  // ceil(range / 19) for range <= 255
+    m = m > 1u ? m : 1u;
+    int32_t b = (int32_t)(a.mn + 10u * m);
+    if (b > 255) b = (int32_t)a.mx - 9 * (int32_t)m;
+    const int32_t mi = (int32_t)m;
+    const int32_t s[8] = {-10, -8, -5, -2, 1, 4, 7, 9};
+    uint32_t lv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lv[k] = (uint32_t)min(max(b + mi * s[k], 0), 255);
+    uint32_t nthr4[7];
+    midpoints(lv, nthr4);
+    // Selector column groups: even columns (0, 2) and odd ones (1, 3) in the
+    // two 16-bit halves; row 0 ends up in the top bits of each 12-bit group.
+    uint32_t ge = 0, go = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t t = rank4(a.w[r], nthr4);
+        // rank -> index: 0..3 -> 3..0 (negative modifiers), 4..7 unchanged
+        const uint32_t f = (~t >> 2) & 0x01010101u;
+        const uint32_t idx = t ^ (f | (f << 1));
+        ge |= (idx & 0x00070007u) << (9 - 3 * r);
+        go |= ((idx >> 8) & 0x00070007u) << (9 - 3 * r);
+    }
+    // 48 bits, big-endian: column 0 (x = 0) first, 12 bits per column
+    const uint32_t hi = ((ge & 0xFFFu) << 12) | (go & 0xFFFu);  // columns 0, 1
+    const uint32_t lo = ((ge >> 16) << 12) | (go >> 16);        // columns 2, 3
+    const uint32_t head = (uint32_t)b | (((m << 4) | 9u) << 8);
+    return make_uint2(perm(hi, head, 0x05060100u), perm(lo, hi, 0x04050600u));
+}
+
+// DXT5 / ETC2 RGBA8 block: {alpha words, colour words}.
+template <int FMT>
+__device__ __forceinline__ uint4 encode_block16(const uint32_t (&p)[16])
+{
+    if constexpr (FMT == kDXT5) {
+        const uint2 al = encode_bc3_alpha(p), c = encode_dxt1(p);
+        return make_uint4(al.x, al.y, c.x, c.y);
+    } else {
+        const uint2 al = encode_eac_alpha(p), c = encode_etc1(p);
+        return make_uint4(al.x, al.y, c.x, c.y);
+    }
 }
 
 }  // namespace goofy_dev

@@ -212,8 +212,8 @@ static void copy_rows(uint8_t* dst, const uint8_t* src, size_t stride, size_t ro
     });
 }
 
-// fmt: GOOFY_FORMAT_DXT1 or GOOFY_FORMAT_ETC1 into `result`, or kHostDual:
-// DXT1 into `result` and ETC1S into `result2` from one pass over the image.
+// fmt: one format into `result`, or kHostDual: DXT1 into `result` and ETC1S
+// into `result2` from one pass over the image.
 constexpr int kHostDual = 1 << 16;
 
 static int compress_host(int fmt, unsigned char* result, const unsigned char* input, unsigned width,
@@ -221,8 +221,7 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
 {
     clear_error();
     const bool dual = fmt == kHostDual;
-    if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1 && !dual)
-        return fail(GOOFY_ERR_INVALID, "unknown format");
+    if (!format_valid(fmt) && !dual) return fail(GOOFY_ERR_INVALID, "unknown format");
     // Same checks, same order as the reference (GoofyTC/goofy_tc.h:1500-1508).
     if (width % 16 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 16");
     if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
@@ -247,7 +246,8 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
     const unsigned cb = std::min(std::min(chunk_brows, bh), max_rows);
     // dual: a chunk's staging output holds its DXT1 blocks, then its ETC1S blocks
     const size_t outs = dual ? 2 : 1;
-    if ((rc = ensure_ctx(ctx, row_bytes * 4 * cb, outs * bw * cb * 8)) != GOOFY_OK) return rc;
+    const size_t bb = dual ? 8 : block_bytes(fmt);
+    if ((rc = ensure_ctx(ctx, row_bytes * 4 * cb, outs * bw * cb * bb)) != GOOFY_OK) return rc;
     const unsigned nchunks = (bh + cb - 1) / cb;
     const LaunchCfg cfg = auto_cfg((uint64_t)bw * cb, di.num_cus);
     hipError_t e = hipSuccess;
@@ -257,9 +257,9 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
         const unsigned rows = std::min(cb, bh - k * cb);
         hipError_t r = hipEventSynchronize(ctx->out_ready[s]);
         if (r != hipSuccess) return r;
-        const size_t n = (size_t)rows * bw * 8;
-        std::memcpy(result + (size_t)k * cb * bw * 8, ctx->h_out[s], n);
-        if (dual) std::memcpy(result2 + (size_t)k * cb * bw * 8, static_cast<uint8_t*>(ctx->h_out[s]) + n, n);
+        const size_t n = (size_t)rows * bw * bb;
+        std::memcpy(result + (size_t)k * cb * bw * bb, ctx->h_out[s], n);
+        if (dual) std::memcpy(result2 + (size_t)k * cb * bw * bb, static_cast<uint8_t*>(ctx->h_out[s]) + n, n);
         return hipSuccess;
     };
 
@@ -273,7 +273,7 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
                                 ctx->stream)) != hipSuccess)
             break;
         if ((e = hipEventRecord(ctx->h2d_done[s], ctx->stream)) != hipSuccess) break;
-        const size_t n = (size_t)bw * rows * 8;
+        const size_t n = (size_t)bw * rows * bb;
         e = dual ? launch_dual((uint32_t)cfg.wg_waves, ctx->d_in[s], row_bytes, ctx->d_out[s],
                                static_cast<uint8_t*>(ctx->d_out[s]) + n, bw, bw * rows, ctx->stream)
                  : launch_plain(fmt, cfg, ctx->d_in[s], row_bytes, ctx->d_out[s], bw, bw * rows, ctx->stream);
@@ -297,15 +297,16 @@ static int compress_device(int fmt, void* d_result, const void* d_input, unsigne
                            void* ev_stop = nullptr)
 {
     clear_error();
-    if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
+    if (!format_valid(fmt)) return fail(GOOFY_ERR_INVALID, "unknown format");
     if (width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
     if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
     if (width == 0 || height == 0) return GOOFY_OK;
     if (!d_result || !d_input) return fail(GOOFY_ERR_INVALID, "null buffer");
     if (pitch < (size_t)width * 4 || pitch % 16 != 0)
         return fail(GOOFY_ERR_INVALID, "pitch must be >= 4*width and a multiple of 16");
-    if ((uintptr_t)d_input % 16 != 0 || (uintptr_t)d_result % 8 != 0)
-        return fail(GOOFY_ERR_INVALID, "input must be 16-byte and result 8-byte aligned");
+    const size_t bb = block_bytes(fmt);
+    if ((uintptr_t)d_input % 16 != 0 || (uintptr_t)d_result % bb != 0)
+        return fail(GOOFY_ERR_INVALID, "input must be 16-byte and result block-size (8 or 16 byte) aligned");
     const uint32_t bw = width / 4, bh = height / 4;
     const uint32_t max_rows = max_stripe_brows(bw);
     if (max_rows == 0) return fail(GOOFY_ERR_TOO_LARGE, "image too wide (4,194,304 pixels or more per row)");
@@ -328,7 +329,7 @@ static int compress_device(int fmt, void* d_result, const void* d_input, unsigne
         const uint32_t nb = rows * bw;  // <= 0xFFFFFFFF by max_stripe_brows
         const LaunchCfg c = ucfg ? cfg : auto_cfg(nb, num_cus);
         hipError_t e = launch_plain(fmt, c, static_cast<const uint8_t*>(d_input) + (size_t)y0 * 4 * pitch, pitch,
-                                    static_cast<uint8_t*>(d_result) + (size_t)y0 * bw * 8, bw, nb,
+                                    static_cast<uint8_t*>(d_result) + (size_t)y0 * bw * bb, bw, nb,
                                     (hipStream_t)stream, y0 == 0 ? (hipEvent_t)ev_start : nullptr,
                                     y0 + rows == bh ? (hipEvent_t)ev_stop : nullptr);
         if (e != hipSuccess) return fail_hip(e, "encode kernel launch");
@@ -390,6 +391,20 @@ int goofy_hip_compress_etc1(unsigned char* result, const unsigned char* input, u
 {
     return compress_host(GOOFY_FORMAT_ETC1, result, input, width, height, stride);
 }
+
+int goofy_hip_compress_dxt5(unsigned char* result, const unsigned char* input, unsigned int width, unsigned int height,
+                            unsigned int stride)
+{
+    return compress_host(GOOFY_FORMAT_DXT5, result, input, width, height, stride);
+}
+
+int goofy_hip_compress_etc2(unsigned char* result, const unsigned char* input, unsigned int width, unsigned int height,
+                            unsigned int stride)
+{
+    return compress_host(GOOFY_FORMAT_ETC2, result, input, width, height, stride);
+}
+
+unsigned int goofy_format_block_bytes(int format) { return format_valid(format) ? (unsigned)block_bytes(format) : 0u; }
 
 int goofy_hip_compress_device(int format, void* d_result, const void* d_input, unsigned int width,
                               unsigned int height, size_t pitch, void* stream)
@@ -734,7 +749,7 @@ int goofy_synth_fill_host(int kind, uint64_t seed, uint32_t texture_id, uint32_t
                           unsigned int width, unsigned int height, size_t pitch, int threads)
 {
     clear_error();
-    if (kind != 0 && kind != 1) return fail(GOOFY_ERR_INVALID, "unknown synth kind");
+    if (kind < 0 || kind > 2) return fail(GOOFY_ERR_INVALID, "unknown synth kind");
     if (width == 0 || height == 0) return GOOFY_OK;
     if (!dst || pitch < (size_t)width * 4) return fail(GOOFY_ERR_INVALID, "bad destination");
     const goofy_synth::Params P = goofy_synth::make_params((uint32_t)kind, seed, texture_id, noise);
@@ -761,7 +776,7 @@ int goofy_synth_fill_device(int kind, uint64_t seed, uint32_t texture_id, uint32
                             unsigned int width, unsigned int height, size_t pitch, void* stream)
 {
     clear_error();
-    if (kind != 0 && kind != 1) return fail(GOOFY_ERR_INVALID, "unknown synth kind");
+    if (kind < 0 || kind > 2) return fail(GOOFY_ERR_INVALID, "unknown synth kind");
     if (width == 0 || height == 0) return GOOFY_OK;
     if (!d_dst || pitch < (size_t)width * 4 || pitch % 4 != 0 || (uintptr_t)d_dst % 4 != 0)
         return fail(GOOFY_ERR_INVALID, "bad destination");

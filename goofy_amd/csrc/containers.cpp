@@ -4,12 +4,14 @@
 // encode (saveDds / saveKtx, Src/main.cpp:154-220; header structs :78-136):
 //   DDS : "DDS " magic, 124-byte header with flags 0x000A1007, width, height,
 //         depth 1, pitchOrLinearSize = payload bytes, one mip level, a FOURCC
-//         pixel format ("DXT1"), caps fields left zero as the harness does;
+//         pixel format ("DXT1" or "DXT5"), caps fields left zero as the
+//         harness does;
 //   KTX1: the 12-byte identifier, endianness 0x04030201, glTypeSize 1,
-//         glInternalFormat GL_ETC1_RGB8_OES (0x8D64) or
-//         GL_COMPRESSED_RGB_S3TC_DXT1_EXT (0x83F0), base format GL_RGB for ETC1
-//         and GL_RGBA otherwise, one face, one mip level, no key/value data,
-//         then the 32-bit image size and the blocks.
+//         glInternalFormat GL_COMPRESSED_RGB_S3TC_DXT1_EXT (0x83F0),
+//         GL_ETC1_RGB8_OES (0x8D64), GL_COMPRESSED_RGBA_S3TC_DXT5_EXT (0x83F3)
+//         or GL_COMPRESSED_RGBA8_ETC2_EAC (0x9278) (Src/main.cpp:64-73), base
+//         format GL_RGB for ETC1 and GL_RGBA otherwise, one face, one mip
+//         level, no key/value data, then the 32-bit image size and the blocks.
 // All fields little-endian.
 #include <cstdio>
 #include <cstring>
@@ -48,7 +50,7 @@ void write_header(int kind, int fmt, unsigned w, unsigned h, size_t payload, uns
         put32(p + 28, 1);           // mip levels
         put32(p + 76, 32);          // pixel format size
         put32(p + 80, 0x4);         // FOURCC
-        put32(p + 84, 0x31545844u); // "DXT1"
+        put32(p + 84, fmt == GOOFY_FORMAT_DXT5 ? 0x35545844u : 0x31545844u);  // "DXT5" / "DXT1"
         return;
     }
     static const unsigned char id[12] = {0xAB, 0x4B, 0x54, 0x58, 0x20, 0x31, 0x31, 0xBB, 0x0D, 0x0A, 0x1A, 0x0A};
@@ -57,7 +59,8 @@ void write_header(int kind, int fmt, unsigned w, unsigned h, size_t payload, uns
     put32(p + 16, 0);            // glType (compressed)
     put32(p + 20, 1);            // glTypeSize
     put32(p + 24, 0);            // glFormat (compressed)
-    put32(p + 28, fmt == GOOFY_FORMAT_ETC1 ? 0x8D64u : 0x83F0u);
+    static const uint32_t gl_format[4] = {0x83F0u, 0x8D64u, 0x83F3u, 0x9278u};
+    put32(p + 28, gl_format[fmt]);
     put32(p + 32, fmt == GOOFY_FORMAT_ETC1 ? 0x1907u : 0x1908u);
     put32(p + 36, w);
     put32(p + 40, h);
@@ -72,12 +75,13 @@ void write_header(int kind, int fmt, unsigned w, unsigned h, size_t payload, uns
 int check(int kind, int fmt, unsigned w, unsigned h, size_t payload)
 {
     if (kind != GOOFY_CONTAINER_DDS && kind != GOOFY_CONTAINER_KTX) return fail(GOOFY_ERR_INVALID, "unknown container");
-    if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
-    if (kind == GOOFY_CONTAINER_DDS && fmt != GOOFY_FORMAT_DXT1)
-        return fail(GOOFY_ERR_INVALID, "DDS holds DXT1 only (ETC1 goes in KTX)");
+    if (!format_valid(fmt)) return fail(GOOFY_ERR_INVALID, "unknown format");
+    if (kind == GOOFY_CONTAINER_DDS && fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_DXT5)
+        return fail(GOOFY_ERR_INVALID, "DDS holds DXT1 and DXT5 only (ETC formats go in KTX)");
     if (w % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
     if (h % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
-    if (payload != (size_t)(w / 4) * (h / 4) * 8) return fail(GOOFY_ERR_INVALID, "payload is not (w/4)*(h/4)*8 bytes");
+    if (payload != (size_t)(w / 4) * (h / 4) * block_bytes(fmt))
+        return fail(GOOFY_ERR_INVALID, "payload is not (w/4)*(h/4) blocks of the format's size");
     if (payload > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "payload above 4 GiB");
     return GOOFY_OK;
 }
@@ -86,9 +90,9 @@ int check(int kind, int fmt, unsigned w, unsigned h, size_t payload)
 
 extern "C" {
 
-size_t goofy_container_bytes(int kind, unsigned int width, unsigned int height)
+size_t goofy_container_bytes(int kind, int format, unsigned int width, unsigned int height)
 {
-    return header_bytes(kind) + (size_t)(width / 4) * (height / 4) * 8;
+    return header_bytes(kind) + (size_t)(width / 4) * (height / 4) * (format_valid(format) ? block_bytes(format) : 8);
 }
 
 int goofy_container_write(int kind, int format, unsigned int width, unsigned int height, const void* blocks,

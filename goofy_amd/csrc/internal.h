@@ -68,6 +68,10 @@ inline const uint32_t* worklist_tile_begin(const TexDesc* descs, size_t ntex)
     return reinterpret_cast<const uint32_t*>(descs + ntex);
 }
 
+inline bool format_valid(int f) { return f >= GOOFY_FORMAT_DXT1 && f <= GOOFY_FORMAT_ETC2; }
+// Bytes per block: 8 (DXT1, ETC1S) or 16 (DXT5, ETC2 RGBA8).
+inline size_t block_bytes(int f) { return f >= GOOFY_FORMAT_DXT5 ? 16u : 8u; }
+
 inline uint64_t div_magic(uint32_t bw) { return ((1ull << 40) + bw - 1) / bw; }
 
 // True when (i * magic) >> 40 == i / bw for every i < nblocks.

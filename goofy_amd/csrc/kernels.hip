@@ -25,6 +25,7 @@ namespace goofy_impl {
 using goofy_dev::BlockClass;
 using goofy_dev::classify;
 using goofy_dev::encode_block;
+using goofy_dev::encode_block16;
 using goofy_dev::pack_dxt1;
 using goofy_dev::pack_etc1;
 
@@ -46,7 +47,17 @@ __device__ __forceinline__ void store8(uint8_t* p, uint2 v)
     else *reinterpret_cast<u32x2*>(p) = w;
 }
 
+template <bool NT>
+__device__ __forceinline__ void store16(uint8_t* p, uint4 v)
+{
+    const u32x4 w = {v.x, v.y, v.z, v.w};
+    if constexpr (NT) __builtin_nontemporal_store(w, reinterpret_cast<u32x4*>(p));
+    else *reinterpret_cast<u32x4*>(p) = w;
+}
+
 // Encode the tile of 64*BPL blocks starting at block `first` of a texture.
+// DXT1 / ETC1S write 8 bytes per block, DXT5 / ETC2 RGBA8 16 (a wave's
+// results then fill 1 KiB contiguously).
 // All loads of the tile are issued before any arithmetic so that a wave has
 // 64*BPL*64 bytes in flight.
 template <int FMT, int BPL, bool NT>
@@ -76,7 +87,8 @@ __device__ __forceinline__ void encode_tile(const uint8_t* __restrict__ src, uin
             const uint32_t p[16] = {px[k][0].x, px[k][0].y, px[k][0].z, px[k][0].w, px[k][1].x, px[k][1].y,
                                     px[k][1].z, px[k][1].w, px[k][2].x, px[k][2].y, px[k][2].z, px[k][2].w,
                                     px[k][3].x, px[k][3].y, px[k][3].z, px[k][3].w};
-            store8<NT>(dst + (uint64_t)i * 8u, encode_block<FMT>(p));
+            if constexpr (FMT >= goofy_dev::kDXT5) store16<NT>(dst + (uint64_t)i * 16u, encode_block16<FMT>(p));
+            else store8<NT>(dst + (uint64_t)i * 8u, encode_block<FMT>(p));
         }
     }
 }
@@ -243,9 +255,12 @@ hipError_t launch_plain(int fmt, const LaunchCfg& c, const void* src, size_t pit
     const uint32_t wg = (uint32_t)c.wg_waves;
     const dim3 grid((ntiles + wg - 1) / wg), block(64u * wg);
     const uint64_t magic = div_magic(bw);
-    if (fmt == GOOFY_FORMAT_DXT1)
-        return plain_bpl<0>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream, start, stop);
-    return plain_bpl<1>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream, start, stop);
+    switch (fmt) {
+    case GOOFY_FORMAT_DXT1: return plain_bpl<0>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream, start, stop);
+    case GOOFY_FORMAT_ETC1: return plain_bpl<1>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream, start, stop);
+    case GOOFY_FORMAT_DXT5: return plain_bpl<2>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream, start, stop);
+    default: return plain_bpl<3>(c, grid, block, src, pitch, dst, bw, nblocks, magic, stream, start, stop);
+    }
 }
 
 hipError_t launch_dual(uint32_t wg_waves, const void* src, size_t pitch, void* dst_dxt1, void* dst_etc1, uint32_t bw,

@@ -10,6 +10,10 @@
 // Kind 1 ("stress"): 64x64 regions, each drawn from {constant colour,
 //   two-colour checker, saturated primaries at 0/255, uniform u8 noise,
 //   normal-map field (unit xyz vectors mapped to [0,255])}.
+// Kind 2 ("waves + alpha"): the colour of kind 0 and an alpha channel made of
+//   64x64 regions, each opaque (255), a binary cutout (0 or 255 where a
+//   three-sinusoid field crosses its midpoint) or that smooth field itself
+//   plus the same uniform noise, in proportions 1 : 1 : 2.
 #pragma once
 
 #include <stdint.h>
@@ -31,6 +35,8 @@ struct Params {
     uint32_t phase[3][3]; // waves: phase offsets
     uint32_t kx[3][3];    // waves: phase increment per pixel along x (wrapping)
     uint32_t ky[3][3];    // waves: phase increment per pixel along y (wrapping)
+    int32_t aamp[3];      // alpha field (kind 2): amplitudes, phases, increments
+    uint32_t aphase[3], akx[3], aky[3];
 };
 
 // 32-bit integer hash (xorshift-multiply finaliser).
@@ -132,8 +138,23 @@ GOOFY_HD uint32_t pixel_stress(const Params& P, uint32_t x, uint32_t y)
     }
 }
 
+GOOFY_HD uint32_t pixel_alpha(const Params& P, uint32_t x, uint32_t y)
+{
+    const uint32_t cls = (hash3(P.seed, x >> 6, y >> 6, 0xA1FAu) >> 8) & 3u;
+    if (cls == 0) return 255u;
+    int32_t v = 128;
+    for (int k = 0; k < 3; ++k) v += (P.aamp[k] * isin(P.aphase[k] + P.akx[k] * x + P.aky[k] * y)) / 16384;
+    if (cls == 1) return v >= 128 ? 255u : 0u;
+    if (P.noise) {
+        const uint32_t h = hash3(P.seed, x, y, 3u);
+        v += (int32_t)(((h >> 16) * (2u * P.noise + 1u)) >> 16) - (int32_t)P.noise;
+    }
+    return clamp255(v);
+}
+
 GOOFY_HD uint32_t pixel(const Params& P, uint32_t x, uint32_t y)
 {
+    if (P.kind == 2) return (pixel_waves(P, x, y) & 0x00FFFFFFu) | (pixel_alpha(P, x, y) << 24);
     return P.kind == 0 ? pixel_waves(P, x, y) : pixel_stress(P, x, y);
 }
 
@@ -173,6 +194,17 @@ inline Params make_params(uint32_t kind, uint64_t seed, uint32_t tex, uint32_t n
             P.kx[c][k] = (uint32_t)(int32_t)((mag * cs) / 16384);
             P.ky[c][k] = (uint32_t)(int32_t)((mag * sn) / 16384);
         }
+    }
+    // Alpha field, drawn after the colour so that kinds 0 and 1 are unchanged.
+    for (int k = 0; k < 3; ++k) {
+        P.aamp[k] = 20 + (int32_t)(sm.next() % 41u);  // 20..60 each
+        P.aphase[k] = (uint32_t)sm.next();
+        const uint32_t t = 5u * (uint32_t)(sm.next() & 0xFFFFu);
+        const int64_t M = (int64_t)(1u << 29) >> (t >> 16);
+        const int64_t mag = M - ((M >> 1) * (int64_t)(t & 0xFFFFu) >> 16);
+        const uint32_t theta = (uint32_t)sm.next();
+        P.akx[k] = (uint32_t)(int32_t)((mag * isin(theta + 0x40000000u)) / 16384);
+        P.aky[k] = (uint32_t)(int32_t)((mag * isin(theta)) / 16384);
     }
     return P;
 }

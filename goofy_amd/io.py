@@ -4,8 +4,8 @@
   reference harness's loader (loadPngAsRgba8, Src/main.cpp:258-343): any PNG
   colour type becomes 8-bit RGBA, alpha is forced to 255, and widths that
   are not a multiple of 16 / heights not a multiple of 4 are refused.
-* ``container_bytes`` / ``write_container`` -- DDS (DXT1) and KTX1 (ETC1 or
-  DXT1) files with the harness's headers (saveDds / saveKtx,
+* ``container_bytes`` / ``write_container`` -- DDS (DXT1, DXT5) and KTX1 (any
+  format) files with the harness's headers (saveDds / saveKtx,
   Src/main.cpp:154-220), written by libgoofy_hip.so.
 """
 from __future__ import annotations
@@ -20,7 +20,7 @@ from . import _check, _lib
 DDS, KTX = 0, 1
 
 _lib.goofy_container_bytes.restype = ctypes.c_size_t
-_lib.goofy_container_bytes.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.c_uint]
+_lib.goofy_container_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint, ctypes.c_uint]
 _lib.goofy_container_write.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p,
                                        ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
                                        ctypes.POINTER(ctypes.c_size_t)]
@@ -59,7 +59,7 @@ def load_png_rgba8(path: str | Path) -> tuple[np.ndarray, int, int]:
 
 def container_bytes(kind: int, fmt: int, width: int, height: int, blocks: np.ndarray) -> bytes:
     blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
-    cap = int(_lib.goofy_container_bytes(kind, width, height))
+    cap = int(_lib.goofy_container_bytes(kind, fmt, width, height))
     out = np.empty(cap, dtype=np.uint8)
     n = ctypes.c_size_t()
     _check(_lib.goofy_container_write(kind, fmt, width, height, blocks.ctypes.data, blocks.nbytes, out.ctypes.data,

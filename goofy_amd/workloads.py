@@ -10,6 +10,9 @@
                      [64, 4096] (width and height independent), 50/50 formats.
 * ``STRESS_16384`` -- one 16384x16384 RGBA8 texture, pitch 65792 B (a multiple
                      of 256 above 4*width), built from 64x64 stress regions.
+* ``ALPHA_512`` / ``ALPHA_8192`` -- waves + noise 12 with an alpha channel
+                     (64x64 regions opaque, binary cutout or smooth), the
+                     inputs of the DXT5 / ETC2 RGBA8 encoders.
 
 Texture content comes from the library's integer generator (goofy_synth_*),
 so host and device produce identical bytes for the same parameters.
@@ -20,8 +23,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-WAVES, STRESS = 0, 1
-DXT1, ETC1 = 0, 1
+WAVES, STRESS, WAVES_ALPHA = 0, 1, 2
+DXT1, ETC1, DXT5, ETC2 = 0, 1, 2, 3
 
 
 @dataclass(frozen=True)
@@ -47,6 +50,8 @@ class Image:
 IMAGE_512 = Image("waves-512x512", WAVES, 0x601F0001, 0, 12, 512, 512, 2048)
 IMAGE_8192 = Image("waves-8192x8192", WAVES, 0x601F0002, 0, 12, 8192, 8192, 32768)
 STRESS_16384 = Image("stress-16384x16384-pitch65792", STRESS, 0x601F0005, 0, 0, 16384, 16384, 65792)
+ALPHA_512 = Image("waves-alpha-512x512", WAVES_ALPHA, 0x601F0006, 0, 12, 512, 512, 2048)
+ALPHA_8192 = Image("waves-alpha-8192x8192", WAVES_ALPHA, 0x601F0007, 0, 12, 8192, 8192, 32768)
 
 BATCH_SEED = 0x601F0003
 SERVE_SEED = 0x601F0004

@@ -21,7 +21,24 @@ extern "C" {
 
 #define GOOFY_HIP_API __attribute__((visibility("default")))
 
-typedef enum goofy_format { GOOFY_FORMAT_DXT1 = 0, GOOFY_FORMAT_ETC1 = 1 } goofy_format;
+/* DXT1 and ETC1S blocks are 8 bytes.  DXT5 (BC3) and ETC2 RGBA8 (EAC alpha)
+ * blocks are 16: an alpha block, then the DXT1 / ETC1S block of the same
+ * pixels, bit-identical to the 8-byte formats.  The reference has no alpha
+ * encoder (its README names one as the next step, "alpha directly instead of
+ * brightness"); the alpha blocks here put every pixel on its nearest palette
+ * level (BC3: a0 = max, a1 = min alpha; EAC: table 9, multiplier
+ * max(1, ceil((max-min)/19)), base on the minimum or, near 255, the maximum).
+ * The host drop-in, the device entry points, the decoders, quality sums and
+ * containers take all four; batches and the server take DXT1 and ETC1S. */
+typedef enum goofy_format {
+    GOOFY_FORMAT_DXT1 = 0,
+    GOOFY_FORMAT_ETC1 = 1,
+    GOOFY_FORMAT_DXT5 = 2,
+    GOOFY_FORMAT_ETC2 = 3
+} goofy_format;
+
+/* Bytes per 4x4 block of a format (8 or 16; 0 for an unknown format). */
+GOOFY_HIP_API unsigned int goofy_format_block_bytes(int format);
 
 typedef enum goofy_status {
     GOOFY_OK = 0,
@@ -54,11 +71,17 @@ GOOFY_HIP_API int goofy_hip_device_count(void);
  * `result` holds the blocks.  Thread-safe. */
 GOOFY_HIP_API int goofy_hip_compress(int format, unsigned char* result, const unsigned char* input,
                                      unsigned int width, unsigned int height, unsigned int stride);
-/* The same two entry points with the exact CompressFunc_t signature
- * (Src/main.cpp:644), so a harness can swap them for the reference's. */
+/* The same entry points with the exact CompressFunc_t signature
+ * (Src/main.cpp:644), so a harness can swap them for the reference's.  The
+ * DXT5 / ETC2 ones have no reference encoder to replace; the harness's
+ * decompressDXT5 / decompressETC2 (Src/main.cpp:579-631) read their output. */
 GOOFY_HIP_API int goofy_hip_compress_dxt1(unsigned char* result, const unsigned char* input, unsigned int width,
                                           unsigned int height, unsigned int stride);
 GOOFY_HIP_API int goofy_hip_compress_etc1(unsigned char* result, const unsigned char* input, unsigned int width,
+                                          unsigned int height, unsigned int stride);
+GOOFY_HIP_API int goofy_hip_compress_dxt5(unsigned char* result, const unsigned char* input, unsigned int width,
+                                          unsigned int height, unsigned int stride);
+GOOFY_HIP_API int goofy_hip_compress_etc2(unsigned char* result, const unsigned char* input, unsigned int width,
                                           unsigned int height, unsigned int stride);
 
 /* Both formats of a host image from one pass: DXT1 blocks into result_dxt1
@@ -328,11 +351,12 @@ typedef struct goofy_serve_sim_result {
 GOOFY_HIP_API int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_result* result);
 
 /* ---- containers ----------------------------------------------------------------
- * DDS (DXT1) and KTX1 (ETC1 or DXT1) files around (width/4)*(height/4)*8 block
- * bytes, with the headers the reference's harness writes (saveDds / saveKtx,
- * Src/main.cpp:154-220).  Host memory only; no GPU needed. */
+ * DDS (DXT1, DXT5) and KTX1 (any of the four formats) files around
+ * (width/4)*(height/4) blocks, with the headers the reference's harness writes
+ * (saveDds / saveKtx, Src/main.cpp:154-220; format codes :64-73).  Host memory
+ * only; no GPU needed.  goofy_container_bytes: header plus payload size. */
 enum { GOOFY_CONTAINER_DDS = 0, GOOFY_CONTAINER_KTX = 1 };
-GOOFY_HIP_API size_t goofy_container_bytes(int kind, unsigned int width, unsigned int height);
+GOOFY_HIP_API size_t goofy_container_bytes(int kind, int format, unsigned int width, unsigned int height);
 GOOFY_HIP_API int goofy_container_write(int kind, int format, unsigned int width, unsigned int height,
                                         const void* blocks, size_t blocks_bytes, void* out, size_t out_capacity,
                                         size_t* written);

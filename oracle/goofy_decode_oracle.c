@@ -24,7 +24,9 @@
 
 static inline int clamp255(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
 
-void goofy_oracle_decode_bc1(const uint8_t* block, uint8_t* out)
+/* four_only: BC3's colour block, always in 4-colour mode
+ * (Src/decoder.cpp:945-956 decodes it with isDxt1 = false). */
+static void decode_bc1_mode(const uint8_t* block, uint8_t* out, int four_only)
 {
     const unsigned c0 = (unsigned)block[0] | ((unsigned)block[1] << 8);
     const unsigned c1 = (unsigned)block[2] | ((unsigned)block[3] << 8);
@@ -37,8 +39,9 @@ void goofy_oracle_decode_bc1(const uint8_t* block, uint8_t* out)
         pal[e][2] = (int)((b << 3) | (b >> 2));
         pal[e][3] = 255;
     }
+    const int four = four_only || c0 > c1;
     for (int c = 0; c < 3; ++c) {
-        if (c0 > c1) {
+        if (four) {
             pal[2][c] = (2 * pal[0][c] + pal[1][c]) / 3;
             pal[3][c] = (pal[0][c] + 2 * pal[1][c]) / 3;
         } else {
@@ -47,7 +50,7 @@ void goofy_oracle_decode_bc1(const uint8_t* block, uint8_t* out)
         }
     }
     pal[2][3] = 255;
-    pal[3][3] = (c0 > c1) ? 255 : 0;
+    pal[3][3] = four ? 255 : 0;
     const uint32_t idx = (uint32_t)block[4] | ((uint32_t)block[5] << 8) | ((uint32_t)block[6] << 16) |
                          ((uint32_t)block[7] << 24);
     for (int p = 0; p < 16; ++p) {
@@ -96,6 +99,63 @@ void goofy_oracle_decode_etc1(const uint8_t* block, uint8_t* out)
         }
 }
 
+void goofy_oracle_decode_bc1(const uint8_t* block, uint8_t* out) { decode_bc1_mode(block, out, 0); }
+
+/* BC3 alpha: a0 > a1 gives 8 entries, a0, a1 and six at sevenths; otherwise
+ * 6 entries at fifths plus 0 and 255.  Interpolants truncate, as in the
+ * harness's decoder (Src/decoder.cpp:874-896).  48 index bits follow,
+ * little-endian, 3 per pixel in row-major order. */
+static void decode_bc3_alpha(const uint8_t* block, uint8_t* out)
+{
+    const int a0 = block[0], a1 = block[1];
+    int pal[8] = {a0, a1, 0, 0, 0, 0, 0, 0};
+    if (a0 > a1) {
+        for (int i = 1; i < 7; ++i) pal[1 + i] = ((7 - i) * a0 + i * a1) / 7;
+    } else {
+        for (int i = 1; i < 5; ++i) pal[1 + i] = ((5 - i) * a0 + i * a1) / 5;
+        pal[6] = 0;
+        pal[7] = 255;
+    }
+    uint64_t bits = 0;
+    for (int i = 0; i < 6; ++i) bits |= (uint64_t)block[2 + i] << (8 * i);
+    for (int p = 0; p < 16; ++p) out[4 * p + 3] = (uint8_t)pal[(bits >> (3 * p)) & 7u];
+}
+
+/* ETC2 alpha (EAC): base, multiplier (high nibble) and table (low nibble),
+ * then 48 big-endian selector bits, pixel (x, y) at position 4x + y from the
+ * top; alpha = clamp(base + table[sel] * multiplier). */
+static const int kEacTables[16][8] = {
+    {-3, -6, -9, -15, 2, 5, 8, 14}, {-3, -7, -10, -13, 2, 6, 9, 12}, {-2, -5, -8, -13, 1, 4, 7, 12},
+    {-2, -4, -6, -13, 1, 3, 5, 12}, {-3, -6, -8, -12, 2, 5, 7, 11}, {-3, -7, -9, -11, 2, 6, 8, 10},
+    {-4, -7, -8, -11, 3, 6, 7, 10}, {-3, -5, -8, -11, 2, 4, 7, 10}, {-2, -6, -8, -10, 1, 5, 7, 9},
+    {-2, -5, -8, -10, 1, 4, 7, 9},  {-2, -4, -8, -10, 1, 3, 7, 9},  {-2, -5, -7, -10, 1, 4, 6, 9},
+    {-3, -4, -7, -10, 2, 3, 6, 9},  {-1, -2, -3, -10, 0, 1, 2, 9},  {-4, -6, -8, -9, 3, 5, 7, 8},
+    {-3, -5, -7, -9, 2, 4, 6, 8}};
+
+static void decode_eac_alpha(const uint8_t* block, uint8_t* out)
+{
+    const int base = block[0], mult = block[1] >> 4, table = block[1] & 15;
+    uint64_t bits = 0;
+    for (int i = 0; i < 6; ++i) bits = (bits << 8) | block[2 + i];
+    for (int x = 0; x < 4; ++x)
+        for (int y = 0; y < 4; ++y) {
+            const int sel = (int)((bits >> (45 - 3 * (4 * x + y))) & 7u);
+            out[4 * (4 * y + x) + 3] = (uint8_t)clamp255(base + kEacTables[table][sel] * mult);
+        }
+}
+
+void goofy_oracle_decode_bc3(const uint8_t* block, uint8_t* out)
+{
+    decode_bc1_mode(block + 8, out, 1);
+    decode_bc3_alpha(block, out);
+}
+
+void goofy_oracle_decode_etc2(const uint8_t* block, uint8_t* out)
+{
+    goofy_oracle_decode_etc1(block + 8, out);
+    decode_eac_alpha(block, out);
+}
+
 int goofy_oracle_decode_image(int fmt, uint8_t* dst, const uint8_t* blocks, unsigned width, unsigned height)
 {
     if (width % 4 != 0) return -1;
@@ -104,9 +164,11 @@ int goofy_oracle_decode_image(int fmt, uint8_t* dst, const uint8_t* blocks, unsi
     uint8_t px[64];
     for (unsigned by = 0; by < bh; ++by)
         for (unsigned bx = 0; bx < bw; ++bx) {
-            const uint8_t* b = blocks + ((size_t)by * bw + bx) * 8;
+            const uint8_t* b = blocks + ((size_t)by * bw + bx) * goofy_oracle_block_bytes(fmt);
             if (fmt == GOOFY_ORACLE_DXT1) goofy_oracle_decode_bc1(b, px);
-            else goofy_oracle_decode_etc1(b, px);
+            else if (fmt == GOOFY_ORACLE_ETC1) goofy_oracle_decode_etc1(b, px);
+            else if (fmt == GOOFY_ORACLE_DXT5) goofy_oracle_decode_bc3(b, px);
+            else goofy_oracle_decode_etc2(b, px);
             for (int r = 0; r < 4; ++r)
                 memcpy(dst + ((size_t)by * 4 + r) * width * 4 + (size_t)bx * 16, px + 16 * r, 16);
         }

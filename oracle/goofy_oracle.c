@@ -112,7 +112,7 @@ void goofy_oracle_analyse_block(const uint8_t* px, goofy_oracle_block_info* info
     info->gez = gez; info->lqt = lqt;
 }
 
-void goofy_oracle_encode_block(int fmt, const uint8_t* px, uint8_t* out)
+static void encode_colour(int fmt, const uint8_t* px, uint8_t* out)
 {
     goofy_oracle_block_info bi;
     goofy_oracle_analyse_block(px, &bi);
@@ -179,6 +179,99 @@ void goofy_oracle_encode_block(int fmt, const uint8_t* px, uint8_t* out)
     put_le32(out + 4, w1);
 }
 
+/* ---- alpha blocks (DXT5 / ETC2 RGBA8) ------------------------------------- */
+
+static void alpha_range(const uint8_t* px, uint8_t a[16], int* amin, int* amax)
+{
+    *amin = 255;
+    *amax = 0;
+    for (int p = 0; p < 16; ++p) {
+        a[p] = px[4 * p + 3];
+        if (a[p] < *amin) *amin = a[p];
+        if (a[p] > *amax) *amax = a[p];
+    }
+}
+
+/* Rank of alpha value v among the sorted palette lv[0..7]: the number of
+ * midpoints ceil((lv[k-1] + lv[k]) / 2), k = 1..7, that v reaches. */
+static int nearest_rank(const int lv[8], int v)
+{
+    int t = 0;
+    for (int k = 1; k < 8; ++k) t += v >= ((lv[k - 1] + lv[k] + 1) >> 1);
+    return t;
+}
+
+void goofy_oracle_alpha_bc3(const uint8_t* px, uint8_t* out)
+{
+    uint8_t a[16];
+    int amin, amax;
+    alpha_range(px, a, &amin, &amax);
+    const int range = amax - amin;
+    /* Palette as the BC3 decoder forms it with a0 = amax > a1 = amin
+     * (Src/decoder.cpp:874-896): entry 0 = a0, 1 = a1, 1+i = ((7-i) a0 +
+     * i a1) / 7 truncated.  In ascending order, rank k holds amin +
+     * floor(k * range / 7), which is entry 1 for k = 0, 8 - k for k = 1..6 and
+     * 0 for k = 7. */
+    int lv[8];
+    for (int k = 0; k < 8; ++k) lv[k] = amin + (k * range) / 7;
+    uint64_t bits = 0;
+    for (int p = 0; p < 16; ++p) {
+        const int t = nearest_rank(lv, a[p]);
+        const int idx = t == 7 ? 0 : (t == 0 ? 1 : 8 - t);
+        bits |= (uint64_t)idx << (3 * p); /* pixel p = 4*row + col, LSB first */
+    }
+    out[0] = (uint8_t)amax;
+    out[1] = (uint8_t)amin;
+    for (int i = 0; i < 6; ++i) out[2 + i] = (uint8_t)(bits >> (8 * i));
+}
+
+/* EAC modifier table 9 (the ETC2 alpha tables, Src/decoder.cpp:25-42), in
+ * ascending order, and the selector of each rank. */
+static const int kEac9Sorted[8] = {-10, -8, -5, -2, 1, 4, 7, 9};
+static const int kEac9Index[8] = {3, 2, 1, 0, 4, 5, 6, 7};
+
+void goofy_oracle_alpha_eac(const uint8_t* px, uint8_t* out)
+{
+    uint8_t a[16];
+    int amin, amax;
+    alpha_range(px, a, &amin, &amax);
+    const int range = amax - amin;
+    const int m = range ? (range + 18) / 19 : 1; /* 19 m >= range; m <= 14 */
+    int b = amin + 10 * m;
+    if (b > 255) b = amax - 9 * m; /* then b >= 0 (m = 14 needs amin <= 8) */
+    int lv[8];
+    for (int k = 0; k < 8; ++k) {
+        const int v = b + m * kEac9Sorted[k];
+        lv[k] = v < 0 ? 0 : (v > 255 ? 255 : v);
+    }
+    /* 48 selector bits, big-endian, pixel (x, y) at position 4x + y from the
+     * top (Src/decoder.cpp:679-713). */
+    uint64_t bits = 0;
+    for (int x = 0; x < 4; ++x)
+        for (int y = 0; y < 4; ++y) {
+            const int idx = kEac9Index[nearest_rank(lv, a[4 * y + x])];
+            bits |= (uint64_t)idx << (45 - 3 * (4 * x + y));
+        }
+    out[0] = (uint8_t)b;
+    out[1] = (uint8_t)((m << 4) | 9);
+    for (int i = 0; i < 6; ++i) out[2 + i] = (uint8_t)(bits >> (40 - 8 * i));
+}
+
+unsigned goofy_oracle_block_bytes(int fmt) { return fmt >= GOOFY_ORACLE_DXT5 ? 16u : 8u; }
+
+void goofy_oracle_encode_block(int fmt, const uint8_t* px, uint8_t* out)
+{
+    if (fmt == GOOFY_ORACLE_DXT5) {
+        goofy_oracle_alpha_bc3(px, out);
+        encode_colour(GOOFY_ORACLE_DXT1, px, out + 8);
+    } else if (fmt == GOOFY_ORACLE_ETC2) {
+        goofy_oracle_alpha_eac(px, out);
+        encode_colour(GOOFY_ORACLE_ETC1, px, out + 8);
+    } else {
+        encode_colour(fmt, px, out);
+    }
+}
+
 /* ---- image drivers ------------------------------------------------------- */
 
 static void gather_block(const uint8_t* src, size_t stride, unsigned bx, unsigned by, uint8_t* px)
@@ -193,11 +286,12 @@ int goofy_oracle_compress_blocks(int fmt, uint8_t* dst, const uint8_t* src, unsi
     if (width % 4 != 0) return -1;
     if (height % 4 != 0) return -2;
     const unsigned bw = width / 4;
+    const size_t bb = goofy_oracle_block_bytes(fmt);
     uint8_t px[64];
     for (unsigned by = row0; by < row1; ++by)
         for (unsigned bx = 0; bx < bw; ++bx) {
             gather_block(src, stride, bx, by, px);
-            goofy_oracle_encode_block(fmt, px, dst + ((size_t)by * bw + bx) * 8);
+            goofy_oracle_encode_block(fmt, px, dst + ((size_t)by * bw + bx) * bb);
         }
     return 0;
 }

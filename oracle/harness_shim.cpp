@@ -8,7 +8,7 @@
 // harness's own code:
 //   * PNG ingestion      -- loadPngAsRgba8 / destroyPng   (Src/main.cpp:258-349)
 //   * DDS / KTX1 writers -- saveDds / saveKtx            (Src/main.cpp:154-220)
-//   * image decoders     -- decompressDXT1 / ETC1        (Src/main.cpp:561-613)
+//   * image decoders     -- decompressDXT1 / DXT5 / ETC1 / ETC2 (Src/main.cpp:561-631)
 //   * quality figures    -- getMsePsnr                   (Src/main.cpp:403-469)
 //   * quality floor      -- generateDownsamlpedRgb565Test (Src/main.cpp:501-555)
 // The harness's main() comes along in the same object; nothing calls it.
@@ -43,18 +43,23 @@ GOOFY_HARNESS_API void goofy_harness_rgb565_baseline(const unsigned char* in, un
     generateDownsamlpedRgb565Test(in, width, height, out);
 }
 
-// fmt 0 = DXT1, 1 = ETC1.  Whole image, tight rows, blocks row-major.
+// fmt 0 = DXT1, 1 = ETC1, 2 = DXT5, 3 = ETC2 RGBA8.  Whole image, tight rows,
+// blocks row-major.
 GOOFY_HARNESS_API void goofy_harness_decompress(int fmt, const unsigned char* blocks, unsigned width, unsigned height,
                                                 unsigned char* rgba8)
 {
     if (fmt == 0) decompressDXT1(blocks, width, height, rgba8);
-    else decompressETC1(blocks, width, height, rgba8);
+    else if (fmt == 1) decompressETC1(blocks, width, height, rgba8);
+    else if (fmt == 2) decompressDXT5(blocks, width, height, rgba8);
+    else decompressETC2(blocks, width, height, rgba8);
 }
 
-// container 0 = DDS (FourCC DXT1), 1 = KTX1 (fmt 0 = DXT1, 1 = ETC1).
+// container 0 = DDS (FourCC DXT1, or DXT5 for fmt 2), 1 = KTX1 (fmt 0 = DXT1,
+// 1 = ETC1, 2 = DXT5, 3 = ETC2 RGBA8).
 GOOFY_HARNESS_API void goofy_harness_save(int container, int fmt, const char* path, const unsigned char* data,
                                           size_t size, unsigned width, unsigned height)
 {
-    if (container == 0) saveDds(path, data, size, width, height, kDdsFormatDXT1);
-    else saveKtx(path, data, size, width, height, fmt == 0 ? kKtxFormatDXT1 : kKtxFormatETC1);
+    static const uint32_t ktx[4] = {kKtxFormatDXT1, kKtxFormatETC1, kKtxFormatDXT5, kKtxFormatETC2};
+    if (container == 0) saveDds(path, data, size, width, height, fmt == 2 ? kDdsFormatDXT5 : kDdsFormatDXT1);
+    else saveKtx(path, data, size, width, height, ktx[fmt & 3]);
 }

@@ -29,7 +29,12 @@ HARNESS_SO = HERE / "_ref" / "libgoofy_harness.so"
 MSE_PSNR_FIELDS = ("mse_r", "mse_g", "mse_b", "mse_a", "mse_max", "mse_rgb", "mse_y",
                    "psnr_r", "psnr_g", "psnr_b", "psnr_a", "psnr_min", "psnr_rgb", "psnr_y")
 
-DXT1, ETC1 = 0, 1
+DXT1, ETC1, DXT5, ETC2 = 0, 1, 2, 3
+
+
+def block_bytes(fmt: int) -> int:
+    """8 for DXT1 / ETC1S, 16 for DXT5 / ETC2 RGBA8 (alpha block + colour block)."""
+    return 16 if fmt in (DXT5, ETC2) else 8
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 
 
@@ -67,6 +72,10 @@ class Oracle:
                                                  ctypes.c_size_t, ctypes.c_int]
         lib.goofy_oracle_decode_bc1.argtypes = [_u8p, _u8p]
         lib.goofy_oracle_decode_etc1.argtypes = [_u8p, _u8p]
+        lib.goofy_oracle_decode_bc3.argtypes = [_u8p, _u8p]
+        lib.goofy_oracle_decode_etc2.argtypes = [_u8p, _u8p]
+        lib.goofy_oracle_alpha_bc3.argtypes = [_u8p, _u8p]
+        lib.goofy_oracle_alpha_eac.argtypes = [_u8p, _u8p]
         lib.goofy_oracle_decode_image.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.c_uint, ctypes.c_uint]
         lib.goofy_oracle_rgb565_baseline.argtypes = [_u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t, ctypes.c_uint,
                                                      ctypes.c_uint]
@@ -82,15 +91,24 @@ class Oracle:
 
     # -- blocks ---------------------------------------------------------------
     def encode_blocks(self, fmt: int, blocks: np.ndarray) -> np.ndarray:
-        """blocks: (n, 64) uint8 (4 rows x 4 RGBA pixels) -> (n, 8) uint8."""
+        """blocks: (n, 64) uint8 (4 rows x 4 RGBA pixels) -> (n, 8 or 16) uint8."""
         blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)
         n = blocks.shape[0]
         # lay blocks side by side in a 4-row image and use the block driver
         img = np.ascontiguousarray(blocks.reshape(n, 4, 16).transpose(1, 0, 2)).reshape(4, n * 16)
-        out = np.zeros((n, 8), dtype=np.uint8)
+        out = np.zeros((n, block_bytes(fmt)), dtype=np.uint8)
         if n:
             rc = self.lib.goofy_oracle_compress_blocks(fmt, _ptr(out), _ptr(img), n * 4, 4, n * 16, 0, 1)
             assert rc == 0
+        return out
+
+    def alpha_blocks(self, fmt: int, blocks: np.ndarray) -> np.ndarray:
+        """The 8-byte alpha blocks alone (fmt DXT5 -> BC3, ETC2 -> EAC) of (n, 64) blocks."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)
+        out = np.zeros((blocks.shape[0], 8), dtype=np.uint8)
+        fn = self.lib.goofy_oracle_alpha_bc3 if fmt == DXT5 else self.lib.goofy_oracle_alpha_eac
+        for i in range(blocks.shape[0]):
+            fn(_ptr(blocks[i]), _ptr(out[i]))
         return out
 
     def analyse(self, block: np.ndarray) -> dict:
@@ -106,14 +124,15 @@ class Oracle:
     # -- images -----------------------------------------------------------------
     def compress(self, fmt: int, img: np.ndarray, width: int, height: int, stride: int) -> tuple[int, np.ndarray]:
         """Drop-in semantics (-1 for width % 16, -2 for height % 4)."""
-        out = np.zeros(max(1, (width // 4) * (height // 4) * 8), dtype=np.uint8)
+        n = (width // 4) * (height // 4) * block_bytes(fmt)
+        out = np.zeros(max(1, n), dtype=np.uint8)
         rc = self.lib.goofy_oracle_compress(fmt, _ptr(out), _ptr(img), width, height, stride)
-        return rc, out[: (width // 4) * (height // 4) * 8]
+        return rc, out[:n]
 
     def compress_blocks(self, fmt: int, img: np.ndarray, width: int, height: int, stride: int,
                         threads: int = 0) -> np.ndarray:
         """Per-block semantics for width % 4 == 0 (multi-threaded)."""
-        n = (width // 4) * (height // 4) * 8
+        n = (width // 4) * (height // 4) * block_bytes(fmt)
         out = np.zeros(max(1, n), dtype=np.uint8)
         threads = threads or min(16, os.cpu_count() or 1)
         rc = self.lib.goofy_oracle_compress_mt(fmt, _ptr(out), _ptr(img), width, height, stride, threads)
@@ -122,9 +141,10 @@ class Oracle:
         return out[:n]
 
     def decode_blocks(self, fmt: int, blocks: np.ndarray) -> np.ndarray:
-        blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 8)
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, block_bytes(fmt))
         out = np.zeros((blocks.shape[0], 64), dtype=np.uint8)
-        fn = self.lib.goofy_oracle_decode_bc1 if fmt == DXT1 else self.lib.goofy_oracle_decode_etc1
+        fn = {DXT1: self.lib.goofy_oracle_decode_bc1, ETC1: self.lib.goofy_oracle_decode_etc1,
+              DXT5: self.lib.goofy_oracle_decode_bc3, ETC2: self.lib.goofy_oracle_decode_etc2}[fmt]
         for i in range(blocks.shape[0]):
             fn(_ptr(blocks[i]), _ptr(out[i]))
         return out

@@ -6,7 +6,10 @@ the encoder against the reference.
 * ramps     -- per-channel gradients with a little noise;
 * threshold -- grey blocks whose lumas sit on eighths of a random range, so
                pixels land on, just inside and just outside the >= mid and
-               near/far boundaries of every range.
+               near/far boundaries of every range;
+* alpha     -- random colour with alpha of six kinds per block: uniform bytes,
+               uniform within a random [lo, hi], 0/255 cutout, constant, ramp
+               lo..hi, and lo with a little noise (the DXT5 / ETC2 inputs).
 
 Blocks are generated in chunks with small integer types, so that images of
 millions of blocks stay within a few hundred MiB.
@@ -15,7 +18,7 @@ from __future__ import annotations
 
 import numpy as np
 
-KINDS = ("uniform", "narrow", "ramps", "threshold")
+KINDS = ("uniform", "narrow", "ramps", "threshold", "alpha")
 _CHUNK = 1 << 18  # blocks per generation step
 
 
@@ -38,9 +41,27 @@ def _blocks(kind: str, rng: np.random.Generator, n: int) -> np.ndarray:
         g = lo + r * rng.integers(0, 9, (n, 16), dtype=np.int32) // 8
         px = np.repeat(g[..., None], 4, axis=2)
         px[..., 3] = rng.integers(0, 256, (n, 16), dtype=np.int32)  # alpha never reaches the output
+    elif kind == "alpha":
+        px = rng.integers(0, 256, (n, 16, 4), dtype=np.int32)
+        k = rng.integers(0, 6, (n, 1))
+        lo = rng.integers(0, 256, (n, 1), dtype=np.int32)
+        hi = rng.integers(0, 256, (n, 1), dtype=np.int32)
+        lo, hi = np.minimum(lo, hi), np.maximum(lo, hi)
+        a = px[:, :, 3]
+        a = np.where(k == 1, lo + rng.integers(0, 1 << 16, (n, 16)) % (hi - lo + 1), a)
+        a = np.where(k == 2, np.where(rng.random((n, 16)) < 0.5, 0, 255), a)
+        a = np.where(k == 3, lo, a)
+        a = np.where(k == 4, lo + (hi - lo) * np.arange(16)[None, :] // 15, a)
+        a = np.where(k == 5, lo + rng.integers(-6, 7, (n, 16)), a)
+        px[:, :, 3] = a
     else:
         raise ValueError(kind)
     return np.clip(px, 0, 255).astype(np.uint8)
+
+
+def random_blocks(kind: str, seed: int, n: int) -> np.ndarray:
+    """(n, 64) uint8 blocks of one kind, each 16 RGBA8 pixels in row-major order."""
+    return _blocks(kind, np.random.default_rng(seed), n).reshape(n, 64)
 
 
 def block_image(kind: str, seed: int, bw: int, bh: int) -> np.ndarray:

@@ -20,10 +20,16 @@ and records what it outputs:
                   pitch-padded 16384^2 stress images, and of a 16-texture
                   batch with widths that are multiples of 4 but not of 16
                   (encoded by the reference on rows padded to a multiple of
-                  16 pixels, then cropped block-wise).
+                  16 pixels, then cropped block-wise); under "alpha_images",
+                  the DXT5 / ETC2 RGBA8 encodings of the images with an alpha
+                  channel (the reference has no alpha encoder: their alpha
+                  blocks come from the oracle, whose rule tests/test_alpha.py
+                  pins through the reference's decoders, and their colour
+                  halves are checked against the reference's DXT1 / ETC1S
+                  here before the digests are written).
 
 Inputs come from the library's seeded integer generator; the input digests
-pin that generator too.  Run: python tests/golden/make_golden.py [real]
+pin that generator too.  Run: python tests/golden/make_golden.py [real|alpha]
 """
 from __future__ import annotations
 
@@ -39,7 +45,7 @@ sys.path.insert(0, str(ROOT))
 
 import goofy_amd  # noqa: E402
 from goofy_amd import workloads as W  # noqa: E402
-from oracle.oracle import Harness, Reference, aligned_empty  # noqa: E402
+from oracle.oracle import Harness, Oracle, Reference, aligned_empty  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 
@@ -171,6 +177,24 @@ def main() -> int:
     return 0
 
 
+def write_alpha_digests(ref: Reference) -> None:
+    ora = Oracle()
+    path = OUT / "digests.json"
+    digests = json.loads(path.read_text())
+    digests["alpha_images"] = {}
+    for im in (W.ALPHA_512, W.ALPHA_8192):
+        img = goofy_amd.synth_host(im.kind, im.seed, im.texture_id, im.noise, im.width, im.height, im.pitch)
+        entry = {"width": im.width, "height": im.height, "pitch": im.pitch, "input_sha256": sha(img)}
+        for fmt, cfmt, name in ((W.DXT5, W.DXT1, "dxt5"), (W.ETC2, W.ETC1, "etc2")):
+            out = ora.compress_blocks(fmt, img, im.width, im.height, im.pitch)
+            colour = encode_image_ref(ref, cfmt, img, im.width, im.height, im.pitch)
+            assert np.array_equal(out.reshape(-1, 16)[:, 8:].reshape(-1), colour), (im.name, name)
+            entry[f"{name}_sha256"] = sha(out)
+        digests["alpha_images"][im.name] = entry
+        print(im.name, entry["input_sha256"][:16])
+    path.write_text(json.dumps(digests, indent=1) + "\n")
+
+
 def write_real_images(ref: Reference) -> None:
     harness = Harness()
     real = {}
@@ -209,4 +233,9 @@ if __name__ == "__main__":
     if sys.argv[1:] == ["real"]:  # only real_images.json
         write_real_images(Reference())
         sys.exit(0)
-    sys.exit(main())
+    if sys.argv[1:] == ["alpha"]:  # only the "alpha_images" digests
+        write_alpha_digests(Reference())
+        sys.exit(0)
+    rc = main()
+    write_alpha_digests(Reference())
+    sys.exit(rc)

@@ -71,6 +71,21 @@ def test_dropin_return_codes_without_device(fn):
     assert fn(out, img, 64, 0, 256) == 0
 
 
+@pytest.mark.parametrize("fn", [goofy_amd.compressDXT5, goofy_amd.compressETC2])
+def test_alpha_dropin_return_codes_without_device(fn):
+    out = np.zeros(64, np.uint8)
+    img = np.zeros(64 * 64 * 4, np.uint8)
+    assert fn(out, img, 60, 64, 240) == -1
+    assert fn(out, img, 64, 62, 256) == -2
+    assert fn(out, img, 0, 64, 0) == 0
+
+
+def test_format_block_bytes():
+    f = goofy_amd._lib.goofy_format_block_bytes
+    assert [f(k) for k in (0, 1, 2, 3, 4, -1)] == [8, 8, 16, 16, 0, 0]
+    assert [goofy_amd.block_bytes(k) for k in (0, 1, 2, 3)] == [8, 8, 16, 16]
+
+
 def test_dual_dropin_return_codes_without_device():
     out_a, out_b = np.zeros(64, np.uint8), np.zeros(64, np.uint8)
     img = np.zeros(64 * 64 * 4, np.uint8)

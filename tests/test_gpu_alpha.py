@@ -1,0 +1,115 @@
+"""DXT5 (BC3) and ETC2 RGBA8 (EAC alpha) on the GPU, through the C-ABI.
+
+Bit-exact against the CPU oracle, whose alpha rule the CPU tests pin through
+the reference's own decoders (tests/test_alpha.py); the colour half of every
+16-byte block is also checked against the GPU's own DXT1 / ETC1S output.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from goofy_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ALPHA = [(W.DXT5, W.DXT1), (W.ETC2, W.ETC1)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev_buffer(nbytes, fill=None):
+    from goofy_amd.hip import DeviceBuffer
+    return DeviceBuffer(nbytes, 256, fill)
+
+
+def to_host(buf):
+    from goofy_amd import hip
+    hip.synchronize()
+    return buf.download()
+
+
+def upload(a):
+    from goofy_amd.hip import DeviceBuffer
+    return DeviceBuffer.from_array(np.ascontiguousarray(a).reshape(-1), 256)
+
+
+def encode_dev(g, fmt, src, w, h, pitch, config=None):
+    n = (w // 4) * (h // 4) * g.block_bytes(fmt)
+    out = dev_buffer(max(16, n), fill=0xCD)
+    g.compress_device(fmt, out, src, w, h, pitch, config=config)
+    return to_host(out)[:n]
+
+
+@pytest.mark.parametrize("fmt,cfmt", ALPHA)
+@pytest.mark.parametrize("name", ["waves-alpha-512x512", "waves-alpha-8192x8192"])
+def test_alpha_images_vs_oracle(gpu, oracle, name, fmt, cfmt):
+    im = {W.ALPHA_512.name: W.ALPHA_512, W.ALPHA_8192.name: W.ALPHA_8192}[name]
+    img = gpu.synth_host(im.kind, im.seed, im.texture_id, im.noise, im.width, im.height, im.pitch)
+    a = img.reshape(-1, 4)[:, 3]
+    assert (a == 255).any() and (a == 0).any() and ((a > 0) & (a < 255)).mean() > 0.2  # all three region kinds
+    src = dev_buffer(im.pitch * im.height, fill=0)
+    gpu.synth_device(im.kind, im.seed, im.texture_id, im.noise, src, im.width, im.height, im.pitch)
+    say(f"alpha {name} fmt={fmt}")
+    got = encode_dev(gpu, fmt, src, im.width, im.height, im.pitch)
+    want = oracle.compress_blocks(fmt, img, im.width, im.height, im.pitch)
+    assert np.array_equal(got, want)
+    colour = encode_dev(gpu, cfmt, src, im.width, im.height, im.pitch)
+    assert np.array_equal(got.reshape(-1, 16)[:, 8:].reshape(-1), colour)
+
+
+@pytest.mark.parametrize("fmt", [W.DXT5, W.ETC2])
+@pytest.mark.parametrize("kind", ["alpha", "uniform", "threshold"])
+def test_alpha_random_blocks_vs_oracle(gpu, oracle, fmt, kind):
+    from blockgen import block_image
+    bw, bh = 1024, 256  # 262,144 blocks
+    img = block_image(kind, 31 + fmt, bw, bh)
+    src = upload(img)
+    say(f"alpha random blocks {kind} fmt={fmt}")
+    got = encode_dev(gpu, fmt, src, 4 * bw, 4 * bh, 16 * bw)
+    assert np.array_equal(got, oracle.compress_blocks(fmt, img, 4 * bw, 4 * bh, 16 * bw))
+
+
+@pytest.mark.parametrize("bpl,wg,nt,order", [(1, 8, 1, 0), (2, 4, 0, 1), (4, 1, 1, 1), (1, 2, 0, 0)])
+def test_alpha_launch_configs_vs_oracle(gpu, oracle, bpl, wg, nt, order):
+    w, h, pitch = 1028, 260, 1028 * 4 + 48
+    img = gpu.synth_host(W.WAVES_ALPHA, 4321, 0, 25, w, h, pitch, fill=0x77)
+    src = upload(img)
+    cfg = dict(blocks_per_lane=bpl, wg_waves=wg, waves_per_cu=0, nontemporal=nt, tile_order=order)
+    for fmt in (W.DXT5, W.ETC2):
+        say(f"alpha config {cfg} fmt={fmt}")
+        assert np.array_equal(encode_dev(gpu, fmt, src, w, h, pitch, cfg),
+                              oracle.compress_blocks(fmt, img, w, h, pitch))
+
+
+def test_alpha_striped(gpu, oracle):
+    """2^19-block rows: the image runs as stripes of 3 + 1 block rows."""
+    from blockgen import block_image
+    bw, bh = 1 << 19, 4
+    img = block_image("alpha", 78, bw, bh)
+    src = upload(img)
+    for fmt in (W.DXT5, W.ETC2):
+        say(f"alpha striped {4 * bw}x{4 * bh} fmt={fmt}")
+        got = encode_dev(gpu, fmt, src, 4 * bw, 4 * bh, 16 * bw)
+        assert np.array_equal(got, oracle.compress_blocks(fmt, img, 4 * bw, 4 * bh, 16 * bw))
+
+
+@pytest.mark.parametrize("w,h,extra", [(16, 4, 0), (48, 100, 4), (528, 36, 200), (2048, 1028, 16)])
+def test_alpha_dropin_vs_oracle(gpu, oracle, w, h, extra):
+    rng = np.random.default_rng(w + h)
+    stride = w * 4 + extra
+    img = rng.integers(0, 256, stride * h, dtype=np.uint8)
+    img[3::8] = 255  # half the pixels opaque
+    n = (w // 4) * (h // 4) * 16
+    for fmt, fn, cfn in ((W.DXT5, gpu.compressDXT5, gpu.compress_dxt5_fn),
+                         (W.ETC2, gpu.compressETC2, gpu.compress_etc2_fn)):
+        say(f"alpha drop-in fmt={fmt} {w}x{h} stride={stride}")
+        out = np.full(n, 0xCD, np.uint8)
+        assert fn(out, img, w, h, stride) == 0
+        rc, want = oracle.compress(fmt, img, w, h, stride)
+        assert rc == 0 and np.array_equal(out, want)
+        assert np.array_equal(gpu.compress(fmt, img, w, h, stride), want)
+        assert fn(out, img, w - 4, h, stride) == -1 and fn(out, img, w, h - 2, stride) == -2

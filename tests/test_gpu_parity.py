@@ -47,7 +47,8 @@ def encode_dev(g, fmt, src, w, h, pitch, config=None):
 
 # ---- generator ----------------------------------------------------------------------
 
-@pytest.mark.parametrize("kind,w,h,pitch", [(0, 512, 512, 2048), (1, 300, 132, 1280), (0, 1028, 36, 4112)])
+@pytest.mark.parametrize("kind,w,h,pitch", [(0, 512, 512, 2048), (1, 300, 132, 1280), (0, 1028, 36, 4112),
+                                            (2, 520, 260, 2096)])
 def test_synth_device_matches_host(gpu, kind, w, h, pitch):
     say(f"synth kind={kind} {w}x{h} pitch={pitch}")
     host = gpu.synth_host(kind, 99, 4, 17, w, h, pitch, fill=0x5A)

@@ -121,8 +121,11 @@ def test_loader_refuses_what_harness_refuses(harness, tmp_path, w, h):
 
 
 def test_containers_match_harness_writer(harness, tmp_path, golden_img512):
-    for container, fmt in ((gio.DDS, W.DXT1), (gio.KTX, W.DXT1), (gio.KTX, W.ETC1)):
-        blocks = golden_img512["dxt1" if fmt == W.DXT1 else "etc1"]
+    rng = np.random.default_rng(9)
+    alpha_blocks = rng.integers(0, 256, 128 * 128 * 16, dtype=np.uint8)  # any 16-byte blocks
+    for container, fmt in ((gio.DDS, W.DXT1), (gio.KTX, W.DXT1), (gio.KTX, W.ETC1),
+                           (gio.DDS, W.DXT5), (gio.KTX, W.DXT5), (gio.KTX, W.ETC2)):
+        blocks = {W.DXT1: golden_img512["dxt1"], W.ETC1: golden_img512["etc1"]}.get(fmt, alpha_blocks)
         p = tmp_path / f"c{container}{fmt}"
         harness.save(container, fmt, p, blocks, 512, 512)
         assert p.read_bytes() == gio.container_bytes(container, fmt, 512, 512, blocks), (container, fmt)

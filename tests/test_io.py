@@ -76,9 +76,19 @@ def test_dds_header_fields(golden_img512):
     assert data[128:] == blocks.tobytes()
 
 
-@pytest.mark.parametrize("fmt,glfmt,base", [(W.ETC1, 0x8D64, 0x1907), (W.DXT1, 0x83F0, 0x1908)])
+def test_dds_dxt5_fourcc():
+    blocks = np.zeros(128 * 128 * 16, np.uint8)
+    data = gio.container_bytes(gio.DDS, W.DXT5, 512, 512, blocks)
+    assert len(data) == 128 + blocks.size
+    assert struct.unpack("<I", data[20:24])[0] == blocks.size
+    assert struct.unpack("<2I4s", data[76:88]) == (32, 4, b"DXT5")
+
+
+@pytest.mark.parametrize("fmt,glfmt,base", [(W.ETC1, 0x8D64, 0x1907), (W.DXT1, 0x83F0, 0x1908),
+                                            (W.DXT5, 0x83F3, 0x1908), (W.ETC2, 0x9278, 0x1908)])
 def test_ktx_header_fields(golden_img512, fmt, glfmt, base):
-    blocks = golden_img512["etc1" if fmt == W.ETC1 else "dxt1"]
+    blocks = {W.ETC1: golden_img512["etc1"], W.DXT1: golden_img512["dxt1"]}.get(
+        fmt, np.arange(128 * 128 * 16, dtype=np.uint32).astype(np.uint8))
     data = gio.container_bytes(gio.KTX, fmt, 512, 512, blocks)
     assert data[:12] == bytes([0xAB, 0x4B, 0x54, 0x58, 0x20, 0x31, 0x31, 0xBB, 0x0D, 0x0A, 0x1A, 0x0A])
     fields = struct.unpack("<14I", data[12:68])
@@ -93,7 +103,11 @@ def test_container_file_and_errors(tmp_path, golden_img512):
     gio.write_container(p, gio.KTX, W.ETC1, 512, 512, blocks)
     assert p.read_bytes() == gio.container_bytes(gio.KTX, W.ETC1, 512, 512, blocks)
     with pytest.raises(goofy_amd.GoofyError):
-        gio.container_bytes(gio.DDS, W.ETC1, 512, 512, blocks)  # DDS holds DXT1 only
+        gio.container_bytes(gio.DDS, W.ETC1, 512, 512, blocks)  # DDS holds DXT1 and DXT5 only
+    with pytest.raises(goofy_amd.GoofyError):
+        gio.container_bytes(gio.DDS, W.ETC2, 512, 512, np.zeros(2 * blocks.size, np.uint8))
+    with pytest.raises(goofy_amd.GoofyError):
+        gio.container_bytes(gio.KTX, W.DXT5, 512, 512, blocks)  # 8-byte blocks for a 16-byte format
     with pytest.raises(goofy_amd.GoofyError):
         gio.container_bytes(gio.KTX, W.ETC1, 512, 508, blocks)  # payload size mismatch
     with pytest.raises(goofy_amd.GoofyError):
