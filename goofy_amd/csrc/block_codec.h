@@ -340,35 +340,60 @@ struct AlphaRows {
 __device__ __forceinline__ AlphaRows alpha_rows(const uint32_t (&p)[16])
 {
     AlphaRows a;
-    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+    // Alpha is the top byte of a pixel word, so the largest (smallest) word
+    // holds the largest (smallest) alpha: v_max3_u32 / v_min3_u32 chains.
+    uint32_t hi = p[0], lo = p[0];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        // [A0 A1 0 0] | [0 0 A2 A3]
-        a.w[r] = perm(p[4 * r + 1], p[4 * r], 0x0c0c0703u) | perm(p[4 * r + 3], p[4 * r + 2], 0x07030c0cu);
-        const uint32_t e = a.w[r] & 0x00FF00FFu, o = (a.w[r] >> 8) & 0x00FF00FFu;
-        mn = pk_min16(mn, pk_min16(e, o));
-        mx = pk_max16(mx, pk_max16(e, o));
+    for (int i = 1; i < 16; ++i) {
+        hi = max(hi, p[i]);
+        lo = min(lo, p[i]);
     }
-    a.mn = pk_min16(mn, mn >> 16) & 0xFFu;
-    a.mx = pk_max16(mx, mx >> 16) & 0xFFu;
+    a.mx = hi >> 24;
+    a.mn = lo >> 24;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)  // [A0 A1 0 0] | [0 0 A2 A3]
+        a.w[r] = perm(p[4 * r + 1], p[4 * r], 0x0c0c0703u) | perm(p[4 * r + 3], p[4 * r + 2], 0x07030c0cu);
     return a;
 }
 
-// Per byte of w: the number (0..7) of thresholds it reaches.  nthr4[k-1] is
-// ~thr_k replicated in all four bytes; avg(~t, v) >= 128 <=> v >= t.
-__device__ __forceinline__ uint32_t rank4(uint32_t w, const uint32_t (&nthr4)[7])
+// From the inverted ascending levels n_k = 255 - lv[k]: ~thr_k in all four
+// bytes, thr_k = ceil((lv[k-1] + lv[k]) / 2).  255 - ceil(y / 2) =
+// floor((510 - y) / 2), the floor average of n_{k-1} and n_k, which is
+// v_lerp_u8 with a zero rounding operand.
+__device__ __forceinline__ void inv_midpoints(const uint32_t (&nlv)[8], uint32_t (&nthr4)[7])
 {
-    uint32_t t = 0;
+    uint32_t r[8];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) t += (avg4(nthr4[k], w) >> 7) & 0x01010101u;
-    return t;
+    for (int k = 0; k < 8; ++k) r[k] = rep4(nlv[k]);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) nthr4[k - 1] = __builtin_amdgcn_lerp(r[k - 1], r[k], 0u);
 }
 
-// Thresholds between consecutive ascending levels, inverted and replicated.
-__device__ __forceinline__ void midpoints(const uint32_t (&lv)[8], uint32_t (&nthr4)[7])
+// Rank bits of four alphas.  m[k] has bit 7 of byte c set when alpha c
+// reaches thr_{k+1}; the thresholds ascend, so the seven flags of a byte are
+// a thermometer code and its count t = sum of flags has the bits
+// t2 = flag 4, t1 = flag 2 ^ flag 4 ^ flag 6, t0 = xor of all seven.
+struct Rank {
+    uint32_t t0, t1, t2;  // bit 7 of byte c: bit j of alpha c's rank
+};
+
+__device__ __forceinline__ Rank rank_bits(uint32_t w, const uint32_t (&nthr4)[7])
 {
+    uint32_t m[7];
 #pragma unroll
-    for (int k = 1; k < 8; ++k) nthr4[k - 1] = rep4(255u - ((lv[k - 1] + lv[k] + 1u) >> 1));
+    for (int k = 0; k < 7; ++k) m[k] = avg4(nthr4[k], w);  // avg(~t, v) >= 128 <=> v >= t
+    Rank r;
+    r.t2 = m[3];
+    r.t1 = m[1] ^ m[3] ^ m[5];
+    r.t0 = m[0] ^ m[2] ^ m[4] ^ m[6] ^ r.t1;
+    return r;
+}
+
+// Index bits (bit 7 of each byte) -> 3-bit index in bits 5..7 of each byte.
+__device__ __forceinline__ uint32_t gather3(uint32_t i0, uint32_t i1, uint32_t i2)
+{
+    const uint32_t x = (i2 & 0x80808080u) | ((i1 >> 1) & 0x40404040u);
+    return (x & 0xC0C0C0C0u) | ((i0 >> 2) & 0x20202020u);
 }
 
 // BC3 alpha block as two little-endian words.
@@ -376,28 +401,32 @@ __device__ __forceinline__ uint2 encode_bc3_alpha(const uint32_t (&p)[16])
 {
     const AlphaRows a = alpha_rows(p);
     // level k = mn + floor(k*range/7); (x * 9363) >> 16 == x / 7 for x <= 1785
-    const uint32_t r9 = (a.mx - a.mn) * 9363u;
-    uint32_t lv[8];
+    const uint32_t range = a.mx - a.mn, r9 = range * 9363u, nmn = 255u - a.mn;
+    uint32_t nlv[8];
+    nlv[0] = nmn;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) lv[k] = a.mn + (((uint32_t)k * r9) >> 16);
+    for (int k = 1; k < 7; ++k) nlv[k] = nmn - (((uint32_t)k * r9) >> 16);
+    nlv[7] = 255u - a.mx;
     uint32_t nthr4[7];
-    midpoints(lv, nthr4);
-    uint32_t lo = 0, hi = 0;
+    inv_midpoints(nlv, nthr4);
+    uint32_t v[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const uint32_t t = rank4(a.w[r], nthr4);
-        // rank -> index: 0 -> 1 (a1), 7 -> 0 (a0), k -> 8 - k in between
-        const uint32_t u = (0x08080808u - t) & 0x07070707u;
-        const uint32_t idx = u ^ (~((u >> 1) | (u >> 2)) & 0x01010101u);
-        // four 3-bit indices (bytes) -> 12 bits, pixel c at bits 3c
-        const uint32_t y = idx | (idx >> 5);
-        const uint32_t v = (y & 0x3Fu) | ((y >> 10) & 0xFC0u);
-        // row r occupies bits 12r..12r+11 of the 48 index bits
-        if (r == 0) lo = v;
-        else if (r == 1) lo |= v << 12;
-        else if (r == 2) { lo |= v << 24; hi = v >> 8; }
-        else hi |= v << 4;
+        const Rank t = rank_bits(a.w[r], nthr4);
+        // rank -> index: 0 -> 1 (a1), 7 -> 0 (a0), k -> 8 - k in between;
+        // 8 - t = -t mod 8 has bits t0, t1^t0, t2^(t1|t0), and t0 flips
+        // where t0 == t1 == t2 (t = 0 or 7)
+        const uint32_t i0 = t.t0 ^ ~((t.t0 ^ t.t1) | (t.t1 ^ t.t2));
+        const uint32_t i1 = t.t1 ^ t.t0;
+        const uint32_t i2 = t.t2 ^ (t.t1 | t.t0);
+        // index of column c in bits 8c+5..8c+7 -> 12 bits, column c at 3c
+        const uint32_t z = gather3(i0, i1, i2);
+        const uint32_t y = z | (z >> 5);
+        v[r] = ((y >> 5) & 0x3Fu) | (((y >> 21) & 0x3Fu) << 6);
     }
+    // row r at bits 12r..12r+11 of the 48 index bits (after a0, a1)
+    const uint32_t lo = v[0] | (v[1] << 12) | (v[2] << 24);
+    const uint32_t hi = (v[2] >> 8) | (v[3] << 4);
     return make_uint2(a.mx | (a.mn << 8) | (lo << 16), (lo >> 16) | (hi << 16));
 }
 
@@ -410,24 +439,26 @@ __device__ __forceinline__ uint2 encode_eac_alpha(const uint32_t (&p)[16])
     m = m > 1u ? m : 1u;
     int32_t b = (int32_t)(a.mn + 10u * m);
     if (b > 255) b = (int32_t)a.mx - 9 * (int32_t)m;
-    const int32_t mi = (int32_t)m;
+    const int32_t nb = 255 - b, mi = (int32_t)m;
     const int32_t s[8] = {-10, -8, -5, -2, 1, 4, 7, 9};
-    uint32_t lv[8];
+    uint32_t nlv[8];  // 255 - clamp(b + m*s) = clamp(255 - b - m*s)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) lv[k] = (uint32_t)min(max(b + mi * s[k], 0), 255);
+    for (int k = 0; k < 8; ++k) nlv[k] = (uint32_t)min(max(nb - mi * s[k], 0), 255);
     uint32_t nthr4[7];
-    midpoints(lv, nthr4);
+    inv_midpoints(nlv, nthr4);
     // Selector column groups: even columns (0, 2) and odd ones (1, 3) in the
-    // two 16-bit halves; row 0 ends up in the top bits of each 12-bit group.
+    // two 16-bit halves, 12 bits each, row 0 in the top three.
     uint32_t ge = 0, go = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const uint32_t t = rank4(a.w[r], nthr4);
+        const Rank t = rank_bits(a.w[r], nthr4);
         // rank -> index: 0..3 -> 3..0 (negative modifiers), 4..7 unchanged
-        const uint32_t f = (~t >> 2) & 0x01010101u;
-        const uint32_t idx = t ^ (f | (f << 1));
-        ge |= (idx & 0x00070007u) << (9 - 3 * r);
-        go |= ((idx >> 8) & 0x00070007u) << (9 - 3 * r);
+        const uint32_t z = gather3(~(t.t0 ^ t.t2), ~(t.t1 ^ t.t2), t.t2);
+        // column 0 / 2 index at bits 5..7 / 21..23 -> 9-3r / 25-3r;
+        // column 1 / 3 at bits 13..15 / 29..31 -> 9-3r / 25-3r
+        const uint32_t e = z & 0x00E000E0u, o = z & 0xE000E000u;
+        ge |= r < 2 ? e << (4 - 3 * r) : e >> (3 * r - 4);
+        go |= o >> (4 + 3 * r);
     }
     // 48 bits, big-endian: column 0 (x = 0) first, 12 bits per column
     const uint32_t hi = ((ge & 0xFFFu) << 12) | (go & 0xFFFu);  // columns 0, 1

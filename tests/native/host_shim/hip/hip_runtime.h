@@ -1,0 +1,54 @@
+// Host stand-in for <hip/hip_runtime.h>, used only to build the device block
+// codec (goofy_amd/csrc/block_codec.h) as ordinary host C++ for a CPU test.
+// It supplies what that header uses: the qualifiers, the uint2/uint4 vector
+// types, min/max, and the two gfx950 byte instructions written from their
+// documented semantics.
+#pragma once
+
+#include <stdint.h>
+
+#include <algorithm>
+
+#define __device__
+#define __host__
+#define __forceinline__ inline __attribute__((always_inline))
+
+struct uint2 {
+    uint32_t x, y;
+};
+struct uint4 {
+    uint32_t x, y, z, w;
+};
+inline uint2 make_uint2(uint32_t x, uint32_t y) { return uint2{x, y}; }
+inline uint4 make_uint4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return uint4{x, y, z, w}; }
+using std::max;
+using std::min;
+
+// V_PERM_B32: byte i of the result is byte sel_i of the 8-byte value
+// {hi:lo} (0..3 from lo, 4..7 from hi); selector 0x0c gives 0x00.  (Other
+// selector values -- sign replication, 0xff -- are not used by the codec.)
+inline uint32_t goofy_host_perm(uint32_t hi, uint32_t lo, uint32_t sel)
+{
+    const uint64_t v = ((uint64_t)hi << 32) | lo;
+    uint32_t out = 0;
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t s = (sel >> (8 * i)) & 0xFFu;
+        const uint32_t b = s < 8 ? (uint32_t)(v >> (8 * s)) & 0xFFu : (s == 0x0c ? 0u : 0xFFu);
+        out |= b << (8 * i);
+    }
+    return out;
+}
+
+// V_LERP_U8: per byte, (a + b + (bit 0 of c's byte)) >> 1.
+inline uint32_t goofy_host_lerp(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t out = 0;
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t s = ((a >> (8 * i)) & 0xFFu) + ((b >> (8 * i)) & 0xFFu) + ((c >> (8 * i)) & 1u);
+        out |= ((s >> 1) & 0xFFu) << (8 * i);
+    }
+    return out;
+}
+
+#define __builtin_amdgcn_perm goofy_host_perm
+#define __builtin_amdgcn_lerp goofy_host_lerp
