@@ -518,15 +518,15 @@ int goofy_hip_tune_device(int format, void* d_result, const void* d_input, unsig
 static int check_image_device(int fmt, const void* d_img, const void* d_blocks, unsigned width, unsigned height,
                               size_t pitch, bool striped)
 {
-    if (fmt != GOOFY_FORMAT_DXT1 && fmt != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
+    if (!format_valid(fmt)) return fail(GOOFY_ERR_INVALID, "unknown format");
     if (width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
     if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
     if (width == 0 || height == 0) return GOOFY_OK;
     if (!d_img || !d_blocks) return fail(GOOFY_ERR_INVALID, "null buffer");
     if (pitch < (size_t)width * 4 || pitch % 16 != 0)
         return fail(GOOFY_ERR_INVALID, "pitch must be >= 4*width and a multiple of 16");
-    if ((uintptr_t)d_img % 16 != 0 || (uintptr_t)d_blocks % 8 != 0)
-        return fail(GOOFY_ERR_INVALID, "image must be 16-byte and blocks 8-byte aligned");
+    if ((uintptr_t)d_img % 16 != 0 || (uintptr_t)d_blocks % block_bytes(fmt) != 0)
+        return fail(GOOFY_ERR_INVALID, "image must be 16-byte and blocks block-size (8 or 16 byte) aligned");
     const uint64_t bw = width / 4, nb = bw * (height / 4);
     if (striped ? max_stripe_brows(bw) == 0 : (!magic_ok(bw, nb) || nb > 0xFFFFFFFFull))
         return fail(GOOFY_ERR_TOO_LARGE, "image too large");
@@ -542,7 +542,7 @@ int goofy_hip_decode_device(int format, void* d_rgba, const void* d_blocks, unsi
     const uint32_t bw = width / 4, bh = height / 4, max_rows = max_stripe_brows(bw);
     for (uint32_t y0 = 0; y0 < bh;) {  // stripes for images beyond one launch's index range
         const uint32_t rows = std::min(max_rows, bh - y0);
-        hipError_t e = launch_decode(format, static_cast<const uint8_t*>(d_blocks) + (size_t)y0 * bw * 8,
+        hipError_t e = launch_decode(format, static_cast<const uint8_t*>(d_blocks) + (size_t)y0 * bw * block_bytes(format),
                                      static_cast<uint8_t*>(d_rgba) + (size_t)y0 * 4 * pitch, width, rows * 4, pitch,
                                      (hipStream_t)stream);
         if (e != hipSuccess) return fail_hip(e, "decode kernel launch");
@@ -663,7 +663,7 @@ int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_sou
         hipError_t e = hipSuccess;
         for (uint32_t y0 = 0; y0 < bh && e == hipSuccess;) {  // stripes beyond one launch's index range
             const uint32_t rows = std::min(max_rows, bh - y0);
-            e = launch_sqerr(format, static_cast<const uint8_t*>(d_blocks) + (size_t)y0 * bw * 8,
+            e = launch_sqerr(format, static_cast<const uint8_t*>(d_blocks) + (size_t)y0 * bw * block_bytes(format),
                              static_cast<const uint8_t*>(d_source) + (size_t)y0 * 4 * pitch, pitch, width, rows * 4,
                              scratch, y0 > 0, (hipStream_t)stream);
             y0 += rows;

@@ -1,21 +1,13 @@
-// codec_extra.hip -- the other side of the encoder: BC1 / ETC1 block decoders
-// and a fused decode-and-compare kernel that gives the round-trip quality
-// (per-channel squared error, luma error, PSNR) of encoded blocks against
-// their source image.
-//
-// Semantics follow the decoders the reference's quality harness uses:
-// BC1 from Src/decoder.cpp:819-871 (5/6-bit expansion by bit replication,
-// thirds by truncating integer division, 3-colour mode when c0 <= c1 with
-// transparent black), ETC1 from Src/decoder.cpp:388-678 (individual and
-// differential modes, eight intensity tables, MSB/LSB selector planes in
-// column-major pixel order, flip bit).  ETC1 blocks whose differential sums
-// leave 0..31 are ETC2 T/H/planar blocks, which the ETC1 encoder never
-// produces; they are decoded here with 5-bit wrap-around and are outside the
-// parity claim.  The quality figures mirror getMsePsnr (Src/main.cpp:403-469).
+// codec_extra.hip -- the other side of the encoder: the decode kernel of the
+// four formats (per-block decoders in block_decode.h, with the semantics of
+// the reference harness's decoders) and a fused decode-and-compare kernel
+// that gives the round-trip quality (per-channel squared error, luma error,
+// PSNR) of encoded blocks against their source image.  The quality figures
+// mirror getMsePsnr (Src/main.cpp:403-469).
 //
 // Layout as in kernels.hip: one lane per 4x4 block, a wave covers 64
-// consecutive blocks; 8-byte block loads and 16-byte pixel-row accesses are
-// coalesced per wave.
+// consecutive blocks; 8- or 16-byte block loads and 16-byte pixel-row
+// accesses are coalesced per wave.
 //
 // Also here: the harness's quality floor, a 2x-downsampled RGB565 copy of the
 // source (generateDownsamlpedRgb565Test, Src/main.cpp:501-555), and the
@@ -24,9 +16,12 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include "block_decode.h"
 #include "internal.h"
 
 namespace goofy_impl {
+
+using goofy_dev::decode_block;
 
 namespace {
 
@@ -39,110 +34,29 @@ __device__ __forceinline__ uint2 load_block(const uint2* p)
     return make_uint2(v.x, v.y);
 }
 
-__device__ __forceinline__ uint32_t pack_rgba(uint32_t r, uint32_t g, uint32_t b, uint32_t a)
+// A block of format FMT: 8 bytes (words x, y; z, w unused) or 16 bytes
+// (DXT5 / ETC2 RGBA8: alpha block in x, y, colour block in z, w).
+template <int FMT>
+__device__ __forceinline__ uint4 load_block_fmt(const uint8_t* blocks, uint64_t i)
 {
-    return r | (g << 8) | (b << 16) | (a << 24);
-}
-
-// 16 decoded pixels (row-major) of a BC1 block.
-__device__ __forceinline__ void decode_bc1(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
-{
-    const uint32_t c0 = w0 & 0xFFFFu, c1 = w0 >> 16;
-    const uint32_t r0 = (c0 >> 11) & 31u, g0 = (c0 >> 5) & 63u, b0 = c0 & 31u;
-    const uint32_t r1 = (c1 >> 11) & 31u, g1 = (c1 >> 5) & 63u, b1 = c1 & 31u;
-    const uint32_t R0 = (r0 << 3) | (r0 >> 2), G0 = (g0 << 2) | (g0 >> 4), B0 = (b0 << 3) | (b0 >> 2);
-    const uint32_t R1 = (r1 << 3) | (r1 >> 2), G1 = (g1 << 2) | (g1 >> 4), B1 = (b1 << 3) | (b1 >> 2);
-    uint32_t pal[4];
-    pal[0] = pack_rgba(R0, G0, B0, 255u);
-    pal[1] = pack_rgba(R1, G1, B1, 255u);
-    if (c0 > c1) {
-        pal[2] = pack_rgba((2u * R0 + R1) / 3u, (2u * G0 + G1) / 3u, (2u * B0 + B1) / 3u, 255u);
-        pal[3] = pack_rgba((R0 + 2u * R1) / 3u, (G0 + 2u * G1) / 3u, (B0 + 2u * B1) / 3u, 255u);
+    if constexpr (FMT >= 2) {
+        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(blocks + i * 16u));
+        return make_uint4(v.x, v.y, v.z, v.w);
     } else {
-        pal[2] = pack_rgba((R0 + R1) / 2u, (G0 + G1) / 2u, (B0 + B1) / 2u, 255u);
-        pal[3] = 0u;
+        const uint2 b = load_block(reinterpret_cast<const uint2*>(blocks) + i);
+        return make_uint4(b.x, b.y, 0u, 0u);
     }
-#pragma unroll
-    for (int p = 0; p < 16; ++p) {
-        const uint32_t k = (w1 >> (2 * p)) & 3u;
-        const uint32_t lo = (k & 1u) ? pal[1] : pal[0];
-        const uint32_t hi = (k & 1u) ? pal[3] : pal[2];
-        px[p] = (k & 2u) ? hi : lo;
-    }
-}
-
-__device__ __forceinline__ uint32_t clamp_add(uint32_t c, int32_t d)
-{
-    const int32_t v = (int32_t)c + d;
-    return (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
-}
-
-// 16 decoded pixels (row-major) of an ETC1 block; w0/w1 are the block's
-// first/second 4 bytes read as little-endian words.
-__device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
-{
-    const uint32_t hi = __builtin_bswap32(w0), lo = __builtin_bswap32(w1);  // big-endian words
-    const bool diff = (hi >> 1) & 1u, flip = hi & 1u;
-    uint32_t base[2][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const uint32_t field = (hi >> (24 - 8 * c)) & 0xFFu;
-        if (diff) {
-            const uint32_t a5 = field >> 3;
-            const int32_t d = (int32_t)(field & 7u) - ((field & 4u) ? 8 : 0);
-            const uint32_t b5 = (uint32_t)((int32_t)a5 + d) & 31u;
-            base[0][c] = (a5 << 3) | (a5 >> 2);
-            base[1][c] = (b5 << 3) | (b5 >> 2);
-        } else {
-            const uint32_t a4 = field >> 4, b4 = field & 15u;
-            base[0][c] = (a4 << 4) | a4;
-            base[1][c] = (b4 << 4) | b4;
-        }
-    }
-    const uint32_t cw[2] = {(hi >> 5) & 7u, (hi >> 2) & 7u};
-    uint32_t pal[2][4];
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        // ETC1 intensity modifiers {small, large} per table index, one byte each:
-        // {2,8},{5,17},{9,29},{13,42},{18,60},{24,80},{33,106},{47,183}
-        const uint32_t a = (uint32_t)((0x2F2118120D090502ull >> (8 * cw[s])) & 0xFFu);
-        const uint32_t b = (uint32_t)((0xB76A503C2A1D1108ull >> (8 * cw[s])) & 0xFFu);
-        const int32_t mods[4] = {(int32_t)a, (int32_t)b, -(int32_t)a, -(int32_t)b};  // selector 0..3
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            pal[s][k] = pack_rgba(clamp_add(base[s][0], mods[k]), clamp_add(base[s][1], mods[k]),
-                                  clamp_add(base[s][2], mods[k]), 255u);
-    }
-#pragma unroll
-    for (int y = 0; y < 4; ++y)
-#pragma unroll
-        for (int x = 0; x < 4; ++x) {
-            const int k = 4 * x + y;
-            const uint32_t sel = (((lo >> (16 + k)) & 1u) << 1) | ((lo >> k) & 1u);
-            const bool second = flip ? (y >= 2) : (x >= 2);
-            const uint32_t* P = second ? pal[1] : pal[0];
-            const uint32_t l = (sel & 1u) ? P[1] : P[0];
-            const uint32_t h = (sel & 1u) ? P[3] : P[2];
-            px[4 * y + x] = (sel & 2u) ? h : l;
-        }
 }
 
 template <int FMT>
-__device__ __forceinline__ void decode_block(uint2 b, uint32_t (&px)[16])
-{
-    if constexpr (FMT == 0) decode_bc1(b.x, b.y, px);
-    else decode_etc1(b.x, b.y, px);
-}
-
-template <int FMT>
-__global__ __launch_bounds__(256) void k_decode(const uint2* __restrict__ blocks, uint8_t* __restrict__ dst,
+__global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ blocks, uint8_t* __restrict__ dst,
                                                 uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nblocks) return;
     const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
     uint32_t px[16];
-    decode_block<FMT>(load_block(blocks + i), px);
+    decode_block<FMT>(load_block_fmt<FMT>(blocks, i), px);
     uint8_t* row = dst + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -205,7 +119,7 @@ __device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], uns
 // little-endian word and takes bits 16..23 as "red").  Grid-stride over the
 // blocks, one partial per workgroup; k_sums_final adds the partials.
 template <int FMT>
-__global__ __launch_bounds__(256) void k_sqerr(const uint2* __restrict__ blocks, const uint8_t* __restrict__ src,
+__global__ __launch_bounds__(256) void k_sqerr(const uint8_t* __restrict__ blocks, const uint8_t* __restrict__ src,
                                                uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
                                                QualitySums* __restrict__ part)
 {
@@ -215,20 +129,20 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint2* __restrict__ blocks,
     // Software pipeline: the loads of a lane's next block are issued before
     // the current block is decoded and compared, so the reads stay in flight
     // across the arithmetic.
-    auto fetch = [&](uint64_t i, uint2& b, u32x4 (&v)[4]) {
+    auto fetch = [&](uint64_t i, uint4& b, u32x4 (&v)[4]) {
         const uint32_t by = (uint32_t)((i * magic) >> 40), bx = (uint32_t)i - by * bw;
         const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
-        b = load_block(blocks + i);
+        b = load_block_fmt<FMT>(blocks, i);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             v[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
     };
     uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    uint2 nb = make_uint2(0u, 0u);
+    uint4 nb = make_uint4(0u, 0u, 0u, 0u);
     u32x4 nv[4] = {};
     if (i < nblocks) fetch(i, nb, nv);
     while (i < nblocks) {
-        const uint2 b = nb;
+        const uint4 b = nb;
         const u32x4 v[4] = {nv[0], nv[1], nv[2], nv[3]};
         const uint64_t next = i + step;
         if (next < nblocks) fetch(next, nb, nv);
@@ -344,12 +258,14 @@ hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width,
     if (nblocks == 0) return hipSuccess;
     const dim3 grid((nblocks + 255u) / 256u), block(256);
     const uint64_t magic = div_magic(bw);
-    if (fmt == 0)
-        hipLaunchKernelGGL(k_decode<0>, grid, block, 0, stream, (const uint2*)blocks, (uint8_t*)dst,
-                           (uint64_t)pitch, bw, nblocks, magic);
-    else
-        hipLaunchKernelGGL(k_decode<1>, grid, block, 0, stream, (const uint2*)blocks, (uint8_t*)dst,
-                           (uint64_t)pitch, bw, nblocks, magic);
+    const uint8_t* b = static_cast<const uint8_t*>(blocks);
+    uint8_t* d = static_cast<uint8_t*>(dst);
+    switch (fmt) {
+    case 0: hipLaunchKernelGGL(k_decode<0>, grid, block, 0, stream, b, d, (uint64_t)pitch, bw, nblocks, magic); break;
+    case 1: hipLaunchKernelGGL(k_decode<1>, grid, block, 0, stream, b, d, (uint64_t)pitch, bw, nblocks, magic); break;
+    case 2: hipLaunchKernelGGL(k_decode<2>, grid, block, 0, stream, b, d, (uint64_t)pitch, bw, nblocks, magic); break;
+    default: hipLaunchKernelGGL(k_decode<3>, grid, block, 0, stream, b, d, (uint64_t)pitch, bw, nblocks, magic);
+    }
     return hipGetLastError();
 }
 
@@ -361,12 +277,14 @@ hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pit
     QualitySums* out = static_cast<QualitySums*>(scratch);
     QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(scratch) + kQualityPartialsOffset);
     const uint64_t magic = div_magic(bw);
-    if (fmt == 0)
-        hipLaunchKernelGGL(k_sqerr<0>, dim3(g), dim3(256), 0, stream, (const uint2*)blocks, (const uint8_t*)src,
-                           (uint64_t)pitch, bw, nblocks, magic, part);
-    else
-        hipLaunchKernelGGL(k_sqerr<1>, dim3(g), dim3(256), 0, stream, (const uint2*)blocks, (const uint8_t*)src,
-                           (uint64_t)pitch, bw, nblocks, magic, part);
+    const uint8_t* b = static_cast<const uint8_t*>(blocks);
+    const uint8_t* s = static_cast<const uint8_t*>(src);
+    switch (fmt) {
+    case 0: hipLaunchKernelGGL(k_sqerr<0>, dim3(g), dim3(256), 0, stream, b, s, (uint64_t)pitch, bw, nblocks, magic, part); break;
+    case 1: hipLaunchKernelGGL(k_sqerr<1>, dim3(g), dim3(256), 0, stream, b, s, (uint64_t)pitch, bw, nblocks, magic, part); break;
+    case 2: hipLaunchKernelGGL(k_sqerr<2>, dim3(g), dim3(256), 0, stream, b, s, (uint64_t)pitch, bw, nblocks, magic, part); break;
+    default: hipLaunchKernelGGL(k_sqerr<3>, dim3(g), dim3(256), 0, stream, b, s, (uint64_t)pitch, bw, nblocks, magic, part);
+    }
     hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out,
                        accumulate ? 1u : 0u);
     return hipGetLastError();

@@ -155,11 +155,13 @@ GOOFY_HIP_API int goofy_hip_tune_device(int format, void* d_result, const void* 
 
 /* ---- decode and round-trip quality ------------------------------------------
  * Decoders with the semantics of the reference's quality harness
- * (Src/decoder.cpp: BC1 :819-871, ETC1 :388-678).  `d_rgba` receives RGBA8
- * rows of `pitch` bytes (multiple of 16, >= 4*width, 16-byte aligned);
- * `d_blocks` holds (width/4)*(height/4) tightly packed 8-byte blocks.
- * ETC1 blocks whose differential sums leave 0..31 (ETC2 T/H/planar) are not
- * ETC1 and are decoded with 5-bit wrap-around. */
+ * (Src/decoder.cpp: BC1 :819-871, ETC1 :388-678, BC3 alpha :874-925 with the
+ * colour block in 4-colour mode :945-956, EAC alpha :679-713).  `d_rgba`
+ * receives RGBA8 rows of `pitch` bytes (multiple of 16, >= 4*width, 16-byte
+ * aligned); `d_blocks` holds (width/4)*(height/4) tightly packed blocks of the
+ * format's size, aligned to it.  ETC1 blocks whose differential sums leave
+ * 0..31 (ETC2 T/H/planar) are not ETC1 and are decoded with 5-bit
+ * wrap-around; DXT1 and ETC1S decode with alpha 255. */
 GOOFY_HIP_API int goofy_hip_decode_device(int format, void* d_rgba, const void* d_blocks, unsigned int width,
                                           unsigned int height, size_t pitch, void* stream);
 

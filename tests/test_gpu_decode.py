@@ -1,4 +1,5 @@
-"""GPU BC1/ETC1 decoders and the round-trip quality kernel, through the C-ABI.
+"""GPU BC1/ETC1 (and BC3 / ETC2 RGBA8) decoders and the round-trip quality
+kernel, through the C-ABI.
 
 Decoders: bit-exact against the decoder oracle, which is itself pinned to the
 reference harness's decoders (tests/test_oracle.py).  Quality: integer
@@ -46,6 +47,32 @@ def test_decode_matches_oracle(gpu, oracle, fmt):
     assert (got[:, w * 4:] == 0x3C).all()  # padding untouched
 
 
+@pytest.mark.parametrize("fmt", [W.DXT5, W.ETC2])
+def test_decode_alpha_formats_match_oracle_and_harness(gpu, oracle, harness, fmt):
+    """Random alpha blocks (every BC3 mode, every EAC table and multiplier,
+    0 included) with valid colour halves, against the oracle's and the
+    harness's decoders."""
+    from goofy_amd import hip
+    rng = np.random.default_rng(17 + fmt)
+    bw, bh = 96, 40
+    n = bw * bh
+    alpha = rng.integers(0, 256, (n, 8), dtype=np.uint8)
+    colour = rng.integers(0, 256, (n, 8), dtype=np.uint8) if fmt == W.DXT5 else random_etc1(rng, n)
+    blocks = np.concatenate([alpha, colour], axis=1)
+    w, h = 4 * bw, 4 * bh
+    pitch = w * 4 + 32
+    d_blocks = hip.DeviceBuffer.from_array(blocks.reshape(-1))
+    d_img = hip.DeviceBuffer(pitch * h, fill=0x3C)
+    print(f"decode fmt={fmt} {w}x{h} pitch={pitch}", flush=True)
+    gpu.decode_device(fmt, d_img, d_blocks, w, h, pitch)
+    hip.synchronize()
+    got = d_img.download().reshape(h, pitch)
+    want = oracle.decode_image(fmt, blocks.reshape(-1), w, h).reshape(h, w * 4)
+    assert np.array_equal(got[:, : w * 4], want)
+    assert np.array_equal(want.reshape(-1), harness.decompress(fmt, blocks.reshape(-1), w, h))
+    assert (got[:, w * 4:] == 0x3C).all()
+
+
 def reference_quality(src, dec):
     """getMsePsnr (Src/main.cpp:403-469) restated in numpy float64."""
     a = src.reshape(-1, 4).astype(np.float64)
@@ -80,6 +107,29 @@ def test_quality_matches_harness_definition(gpu, oracle, fmt):
     assert q["psnr_rgb"] == pytest.approx(ref["psnr_rgb"], abs=1e-7)
     assert q["psnr_r"] == pytest.approx(ref["psnr_r"], abs=1e-7)
     assert q["mse_a"] == 0.0 and q["psnr_a"] > 1e300  # alpha is 255 in and out
+
+
+@pytest.mark.parametrize("fmt", [W.DXT5, W.ETC2])
+def test_quality_alpha_formats_match_harness(gpu, harness, fmt):
+    """Round trip of the alpha image through the 16-byte formats: the GPU
+    quality call against getMsePsnr of the harness over the harness's own
+    decode of the same blocks."""
+    from goofy_amd import hip
+    im = W.ALPHA_512
+    img = gpu.synth_host(im.kind, im.seed, im.texture_id, im.noise, im.width, im.height, im.pitch)
+    src = hip.DeviceBuffer.from_array(img)
+    blocks = hip.DeviceBuffer(im.blocks * 16)
+    gpu.compress_device(fmt, blocks, src, im.width, im.height, im.pitch)
+    q = gpu.quality_device(fmt, blocks, src, im.width, im.height, im.pitch)
+    dec = harness.decompress(fmt, blocks.download(), im.width, im.height)
+    ref = harness.mse_psnr(img, dec, im.width, im.height)
+    ours = reference_quality(img, dec)
+    print(f"quality fmt={fmt}: psnr_a={q['psnr_a']:.3f} psnr_rgb={q['psnr_rgb']:.3f}", flush=True)
+    assert q["sse_byte"] == ours["sse_byte"] and q["sse_byte"][3] > 0
+    for k in ("mse_r", "mse_g", "mse_b", "mse_a", "mse_rgb", "psnr_a", "psnr_min", "psnr_rgb"):
+        assert q[k] == pytest.approx(ref[k], rel=1e-12), k
+    assert q["mse_y"] == pytest.approx(ref["mse_y"], rel=1e-9)
+    assert q["psnr_a"] > 40.0  # nearest-level alpha: error at most a few units per pixel
 
 
 def test_quality_of_black_image_closed_form(gpu):

@@ -35,23 +35,31 @@ def test_oracle_under_asan_ubsan(tmp_path):
 CLANG = Path("/opt/rocm/llvm/bin/clang++")
 
 
-@pytest.mark.skipif(not CLANG.exists(), reason="clang++ of the ROCm toolchain not available")
-@pytest.mark.parametrize("kind", ["uniform", "narrow", "ramps", "threshold", "alpha"])
-def test_device_block_codec_on_host_matches_oracle(tmp_path, oracle, kind):
-    """goofy_amd/csrc/block_codec.h (the kernels' per-block arithmetic) built
-    for the host, with v_perm_b32 / v_lerp_u8 written from their documented
-    semantics: all four encoders against the oracle, block for block."""
-    import numpy as np
-
-    from blockgen import random_blocks
-    exe = tmp_path / "codec_host"
+@pytest.fixture(scope="module")
+def codec_host(tmp_path_factory):
+    """tests/native/codec_host.cpp: the kernels' per-block encoders and
+    decoders (block_codec.h, block_decode.h) built for the host, with
+    v_perm_b32 / v_lerp_u8 written from their documented semantics."""
+    if not CLANG.exists():
+        pytest.skip("clang++ of the ROCm toolchain not available")
+    exe = tmp_path_factory.mktemp("codec") / "codec_host"
     subprocess.run([str(CLANG), "-std=c++17", "-O2", f"-I{ROOT / 'tests' / 'native' / 'host_shim'}",
                     f"-I{ROOT / 'goofy_amd' / 'csrc'}", str(ROOT / "tests" / "native" / "codec_host.cpp"),
                     "-o", str(exe)], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("kind", ["uniform", "narrow", "ramps", "threshold", "alpha"])
+def test_device_block_codec_on_host_matches_oracle(tmp_path, oracle, codec_host, kind):
+    """All four encoders of the kernels against the oracle, block for block."""
+    import numpy as np
+
+    from blockgen import random_blocks
     n = 200000
     blocks = random_blocks(kind, 99, n)
     (tmp_path / "in.bin").write_bytes(blocks.tobytes())
-    subprocess.run([str(exe), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], check=True, timeout=120)
+    subprocess.run([str(codec_host), "encode", str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], check=True,
+                   timeout=120)
     out = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.uint8)
     sizes = [8 * n, 8 * n, 16 * n, 16 * n]
     parts = np.split(out, np.cumsum(sizes)[:-1])
@@ -59,3 +67,29 @@ def test_device_block_codec_on_host_matches_oracle(tmp_path, oracle, kind):
         want = oracle.encode_blocks(fmt, blocks)
         bad = np.nonzero((got.reshape(n, -1) != want).any(1))[0]
         assert bad.size == 0, f"format {fmt}: {bad.size} blocks differ, first {bad[:5]}"
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3])
+def test_device_block_decoders_on_host_match_oracle(tmp_path, oracle, harness, codec_host, fmt):
+    """The kernels' decoders against the oracle's and the harness's, on every
+    bit pattern of random blocks (ETC1 colour blocks restricted to the
+    individual and overflow-free differential modes the encoder emits)."""
+    import numpy as np
+
+    from blockgen import random_blocks
+    rng = np.random.default_rng(40 + fmt)
+    n = 100000
+    bb = 16 if fmt >= 2 else 8
+    blocks = rng.integers(0, 256, (n, bb), dtype=np.uint8)
+    if fmt in (1, 3):  # encoder output for the ETC colour half, plus individual-mode blocks
+        colour = oracle.encode_blocks(1, random_blocks("uniform", 41, n))
+        colour[::2, 3] &= 0xFD  # clear the diff bit: individual mode
+        blocks[:, bb - 8:] = colour
+    (tmp_path / "in.bin").write_bytes(blocks.tobytes())
+    subprocess.run([str(codec_host), "decode", str(fmt), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                   check=True, timeout=120)
+    got = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.uint8).reshape(n, 64)
+    img = oracle.decode_image(fmt, blocks.reshape(-1), 4 * n, 4).reshape(4, n, 16).transpose(1, 0, 2)
+    assert np.array_equal(got, img.reshape(n, 64))
+    ref = harness.decompress(fmt, blocks.reshape(-1), 4 * n, 4).reshape(4, n, 16).transpose(1, 0, 2)
+    assert np.array_equal(got, ref.reshape(n, 64))
