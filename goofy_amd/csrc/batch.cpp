@@ -41,6 +41,7 @@ struct goofy_batch {
     size_t d_descs_cap = 0;
     uint32_t ntiles = 0;
     uint64_t blocks = 0, pixels = 0;
+    bool alpha = false;  // holds DXT5 / ETC2 RGBA8 textures
     LaunchCfg cfg{};
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -72,7 +73,7 @@ int build_worklist(goofy_batch* b)
             const uint32_t rows = std::min(max_rows, bh - y0);
             TexDesc d{};
             d.src = (uint64_t)(uintptr_t)t.src + (uint64_t)y0 * 4 * t.pitch;
-            d.dst = (uint64_t)(uintptr_t)t.dst + (uint64_t)y0 * bw * 8;
+            d.dst = (uint64_t)(uintptr_t)t.dst + (uint64_t)y0 * bw * block_bytes(t.format);
             d.pitch = t.pitch;
             d.bw = bw;
             d.nblocks = rows * bw;
@@ -119,7 +120,7 @@ int capture(goofy_batch* b)
     e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return fail_hip(e, "hipStreamBeginCapture");
     hipError_t le = launch_batch(b->cfg, b->d_descs, (uint32_t)b->descs.size(), b->ntiles, b->num_cus,
-                                 b->capture_stream);
+                                 b->capture_stream, b->alpha);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(b->capture_stream, &g);
     if (le != hipSuccess) {
@@ -151,12 +152,16 @@ std::map<Key, LaunchCfg>& tune_cache()
 
 Key shape_key(const goofy_batch* b)
 {
-    uint64_t dxt = 0;
-    for (const auto& t : b->textures) dxt += t.format == GOOFY_FORMAT_DXT1;
+    uint64_t dxt = 0, alpha = 0;
+    for (const auto& t : b->textures) {
+        dxt += t.format == GOOFY_FORMAT_DXT1;
+        alpha += t.format >= GOOFY_FORMAT_DXT5;
+    }
     const int n = (int)std::max<size_t>(1, b->textures.size());
     const int lb = (int)std::log2((double)std::max<uint64_t>(1, b->blocks));
     const int lpt = (int)std::log2((double)std::max<uint64_t>(1, b->blocks / (uint64_t)n));
-    return Key{b->arch, lb, lpt, (int)(dxt * 10 / (uint64_t)n)};
+    // share mix: tenths of DXT1 textures, plus 11 x tenths of alpha textures
+    return Key{b->arch, lb, lpt, (int)(dxt * 10 / (uint64_t)n + 11 * (alpha * 10 / (uint64_t)n))};
 }
 
 void load_file_cache()
@@ -196,16 +201,15 @@ int check_texture(const goofy_texture_desc& t, size_t i)
 {
     char where[64];
     std::snprintf(where, sizeof where, "texture %zu: ", i);
-    if (t.format != GOOFY_FORMAT_DXT1 && t.format != GOOFY_FORMAT_ETC1)
-        return fail(GOOFY_ERR_INVALID, std::string(where) + "unknown format");
+    if (!format_valid(t.format)) return fail(GOOFY_ERR_INVALID, std::string(where) + "unknown format");
     if (t.width % 4 != 0) return fail(GOOFY_ERR_WIDTH, std::string(where) + "width must be a multiple of 4");
     if (t.height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, std::string(where) + "height must be a multiple of 4");
     if (t.width == 0 || t.height == 0) return GOOFY_OK;
     if (!t.src || !t.dst) return fail(GOOFY_ERR_INVALID, std::string(where) + "null buffer");
     if (t.pitch < (size_t)t.width * 4 || t.pitch % 16 != 0)
         return fail(GOOFY_ERR_INVALID, std::string(where) + "pitch must be >= 4*width and a multiple of 16");
-    if ((uintptr_t)t.src % 16 != 0 || (uintptr_t)t.dst % 8 != 0)
-        return fail(GOOFY_ERR_INVALID, std::string(where) + "src must be 16-byte and dst 8-byte aligned");
+    if ((uintptr_t)t.src % 16 != 0 || (uintptr_t)t.dst % block_bytes(t.format) != 0)
+        return fail(GOOFY_ERR_INVALID, std::string(where) + "src must be 16-byte and dst block-size aligned");
     if (max_stripe_brows(t.width / 4) == 0)  // larger textures become several work-list entries
         return fail(GOOFY_ERR_TOO_LARGE, std::string(where) + "too wide (4,194,304 pixels or more per row)");
     return GOOFY_OK;
@@ -236,6 +240,7 @@ int goofy_batch_create(const goofy_texture_desc* textures, size_t count, const g
         const goofy_texture_desc& t = textures[i];
         if (t.width == 0 || t.height == 0) continue;
         b->textures.push_back(t);
+        b->alpha |= t.format >= GOOFY_FORMAT_DXT5;
         b->blocks += (uint64_t)(t.width / 4) * (t.height / 4);
         b->pixels += (uint64_t)t.width * t.height;
     }
@@ -336,12 +341,12 @@ int goofy_batch_autotune(goofy_batch* b, void* stream, int reps, goofy_launch_co
                     for (int order : orders) {
                         b->cfg = LaunchCfg{bpl, wg, wpc, nt, order};
                         // one warm-up launch, then the median of `reps` timed ones
-                        e = launch_batch(b->cfg, b->d_descs, (uint32_t)b->descs.size(), b->ntiles, b->num_cus, s);
+                        e = launch_batch(b->cfg, b->d_descs, (uint32_t)b->descs.size(), b->ntiles, b->num_cus, s, b->alpha);
                         if (e != hipSuccess) { rc = fail_hip(e, "autotune launch"); goto done; }
                         std::vector<float> ts;
                         for (int r = 0; r < reps; ++r) {
                             (void)hipEventRecord(e0, s);
-                            e = launch_batch(b->cfg, b->d_descs, (uint32_t)b->descs.size(), b->ntiles, b->num_cus, s);
+                            e = launch_batch(b->cfg, b->d_descs, (uint32_t)b->descs.size(), b->ntiles, b->num_cus, s, b->alpha);
                             (void)hipEventRecord(e1, s);
                             if (e != hipSuccess) { rc = fail_hip(e, "autotune launch"); goto done; }
                             e = hipEventSynchronize(e1);

@@ -97,10 +97,12 @@ hipError_t launch_plain(int fmt, const LaunchCfg& cfg, const void* src, size_t p
                         uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_dual(uint32_t wg_waves, const void* src, size_t pitch, void* dst_dxt1, void* dst_etc1, uint32_t bw,
                        uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+// alpha: the work-list holds DXT5 / ETC2 RGBA8 textures too (the kernel
+// instance with all four formats; without, DXT1 / ETC1S only).
 hipError_t launch_batch(const LaunchCfg& cfg, const TexDesc* d_descs, uint32_t ntex, uint32_t ntiles, int num_cus,
-                        hipStream_t stream);
+                        hipStream_t stream, bool alpha = false);
 hipError_t launch_batch_inline(const LaunchCfg& cfg, const InlineWorklist& wl, uint32_t ntex, uint32_t ntiles,
-                               int num_cus, hipStream_t stream);
+                               int num_cus, hipStream_t stream, bool alpha = false);
 hipError_t launch_synth(const void* params, void* dst, uint32_t width, uint32_t height, size_t pitch,
                         hipStream_t stream);
 hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width, uint32_t height, size_t pitch,

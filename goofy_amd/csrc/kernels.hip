@@ -151,7 +151,9 @@ __global__ __launch_bounds__(512) void k_encode_dual(const uint8_t* __restrict__
 // the wave steps forward across texture boundaries.  tpw = 1 is one wave per
 // tile; tpw > 1 is a persistent grid.  Formats are uniform per tile, so the
 // format branch never diverges.
-template <int BPL, bool NT, class Descs, class Begins>
+// ALPHA: the work-list may hold DXT5 / ETC2 RGBA8 textures as well; batches
+// of DXT1 / ETC1S textures only run the instance without those branches.
+template <int BPL, bool NT, bool ALPHA, class Descs, class Begins>
 __device__ __forceinline__ void batch_body(const Descs& descs, const Begins& tile_begin, uint32_t ntex,
                                            uint32_t ntiles, uint32_t tpw, uint32_t order)
 {
@@ -177,25 +179,34 @@ __device__ __forceinline__ void batch_body(const Descs& descs, const Begins& til
         const uint32_t first = (t - begin) * 64u * BPL;
         const uint8_t* src = reinterpret_cast<const uint8_t*>(D.src);
         uint8_t* dst = reinterpret_cast<uint8_t*>(D.dst);
-        if (D.fmt == 0) encode_tile<0, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane);
-        else encode_tile<1, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane);
+        if constexpr (ALPHA) {
+            switch (D.fmt) {
+            case 0: encode_tile<0, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane); break;
+            case 1: encode_tile<1, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane); break;
+            case 2: encode_tile<2, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane); break;
+            default: encode_tile<3, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane);
+            }
+        } else {
+            if (D.fmt == 0) encode_tile<0, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane);
+            else encode_tile<1, BPL, NT>(src, D.pitch, dst, D.bw, D.magic, D.nblocks, first, lane);
+        }
     }
 }
 
-template <int BPL, bool NT>
+template <int BPL, bool NT, bool ALPHA>
 __global__ __launch_bounds__(512) void k_encode_batch(const TexDesc* __restrict__ descs,
                                                       const uint32_t* __restrict__ tile_begin, uint32_t ntex,
                                                       uint32_t ntiles, uint32_t tpw, uint32_t order)
 {
-    batch_body<BPL, NT>(descs, tile_begin, ntex, ntiles, tpw, order);
+    batch_body<BPL, NT, ALPHA>(descs, tile_begin, ntex, ntiles, tpw, order);
 }
 
 // The same with the work-list in the kernel arguments (small groups).
-template <int BPL, bool NT>
+template <int BPL, bool NT, bool ALPHA>
 __global__ __launch_bounds__(512) void k_encode_batch_inline(const InlineWorklist wl, uint32_t ntex, uint32_t ntiles,
                                                              uint32_t tpw, uint32_t order)
 {
-    batch_body<BPL, NT>(wl.d, wl.tile_begin, ntex, ntiles, tpw, order);
+    batch_body<BPL, NT, ALPHA>(wl.d, wl.tile_begin, ntex, ntiles, tpw, order);
 }
 
 __global__ __launch_bounds__(256) void k_synth(goofy_synth::Params P, uint8_t* __restrict__ dst, uint32_t width,
@@ -279,17 +290,25 @@ hipError_t launch_dual(uint32_t wg_waves, const void* src, size_t pitch, void* d
     return hipGetLastError();
 }
 
-template <int BPL>
-static hipError_t batch_nt(bool nt, dim3 grid, dim3 block, const TexDesc* d, const uint32_t* tb, uint32_t ntex,
+template <int BPL, bool ALPHA>
+static hipError_t batch_go(bool nt, dim3 grid, dim3 block, const TexDesc* d, const uint32_t* tb, uint32_t ntex,
                            uint32_t ntiles, uint32_t tpw, uint32_t order, hipStream_t s)
 {
-    if (nt) hipLaunchKernelGGL((k_encode_batch<BPL, true>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order);
-    else hipLaunchKernelGGL((k_encode_batch<BPL, false>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order);
+    if (nt) hipLaunchKernelGGL((k_encode_batch<BPL, true, ALPHA>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order);
+    else hipLaunchKernelGGL((k_encode_batch<BPL, false, ALPHA>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order);
     return hipGetLastError();
 }
 
+template <int BPL>
+static hipError_t batch_nt(bool nt, bool alpha, dim3 grid, dim3 block, const TexDesc* d, const uint32_t* tb,
+                           uint32_t ntex, uint32_t ntiles, uint32_t tpw, uint32_t order, hipStream_t s)
+{
+    return alpha ? batch_go<BPL, true>(nt, grid, block, d, tb, ntex, ntiles, tpw, order, s)
+                 : batch_go<BPL, false>(nt, grid, block, d, tb, ntex, ntiles, tpw, order, s);
+}
+
 hipError_t launch_batch(const LaunchCfg& c, const TexDesc* d_descs, uint32_t ntex, uint32_t ntiles, int num_cus,
-                        hipStream_t stream)
+                        hipStream_t stream, bool alpha)
 {
     if (ntiles == 0 || ntex == 0) return hipSuccess;
     const uint32_t* tb = worklist_tile_begin(d_descs, ntex);
@@ -300,23 +319,33 @@ hipError_t launch_batch(const LaunchCfg& c, const TexDesc* d_descs, uint32_t nte
     const uint32_t wg = (uint32_t)c.wg_waves;
     const dim3 grid((uint32_t)((waves + wg - 1) / wg)), block(64u * wg);
     switch (c.blocks_per_lane) {
-    case 1: return batch_nt<1>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, c.tile_order, stream);
-    case 2: return batch_nt<2>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, c.tile_order, stream);
-    default: return batch_nt<4>(c.nontemporal, grid, block, d_descs, tb, ntex, ntiles, tpw, c.tile_order, stream);
+    case 1: return batch_nt<1>(c.nontemporal, alpha, grid, block, d_descs, tb, ntex, ntiles, tpw, c.tile_order, stream);
+    case 2: return batch_nt<2>(c.nontemporal, alpha, grid, block, d_descs, tb, ntex, ntiles, tpw, c.tile_order, stream);
+    default: return batch_nt<4>(c.nontemporal, alpha, grid, block, d_descs, tb, ntex, ntiles, tpw, c.tile_order, stream);
     }
 }
 
-template <int BPL>
-static hipError_t batch_inline_nt(bool nt, dim3 grid, dim3 block, const InlineWorklist& wl, uint32_t ntex,
+template <int BPL, bool ALPHA>
+static hipError_t batch_inline_go(bool nt, dim3 grid, dim3 block, const InlineWorklist& wl, uint32_t ntex,
                                   uint32_t ntiles, uint32_t tpw, uint32_t order, hipStream_t s)
 {
-    if (nt) hipLaunchKernelGGL((k_encode_batch_inline<BPL, true>), grid, block, 0, s, wl, ntex, ntiles, tpw, order);
-    else hipLaunchKernelGGL((k_encode_batch_inline<BPL, false>), grid, block, 0, s, wl, ntex, ntiles, tpw, order);
+    if (nt)
+        hipLaunchKernelGGL((k_encode_batch_inline<BPL, true, ALPHA>), grid, block, 0, s, wl, ntex, ntiles, tpw, order);
+    else
+        hipLaunchKernelGGL((k_encode_batch_inline<BPL, false, ALPHA>), grid, block, 0, s, wl, ntex, ntiles, tpw, order);
     return hipGetLastError();
 }
 
+template <int BPL>
+static hipError_t batch_inline_nt(bool nt, bool alpha, dim3 grid, dim3 block, const InlineWorklist& wl, uint32_t ntex,
+                                  uint32_t ntiles, uint32_t tpw, uint32_t order, hipStream_t s)
+{
+    return alpha ? batch_inline_go<BPL, true>(nt, grid, block, wl, ntex, ntiles, tpw, order, s)
+                 : batch_inline_go<BPL, false>(nt, grid, block, wl, ntex, ntiles, tpw, order, s);
+}
+
 hipError_t launch_batch_inline(const LaunchCfg& c, const InlineWorklist& wl, uint32_t ntex, uint32_t ntiles,
-                               int num_cus, hipStream_t stream)
+                               int num_cus, hipStream_t stream, bool alpha)
 {
     if (ntiles == 0 || ntex == 0) return hipSuccess;
     if (ntex > (uint32_t)kInlineTextures) return hipErrorInvalidValue;
@@ -327,9 +356,9 @@ hipError_t launch_batch_inline(const LaunchCfg& c, const InlineWorklist& wl, uin
     const uint32_t wg = (uint32_t)c.wg_waves;
     const dim3 grid((uint32_t)((waves + wg - 1) / wg)), block(64u * wg);
     switch (c.blocks_per_lane) {
-    case 1: return batch_inline_nt<1>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, c.tile_order, stream);
-    case 2: return batch_inline_nt<2>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, c.tile_order, stream);
-    default: return batch_inline_nt<4>(c.nontemporal, grid, block, wl, ntex, ntiles, tpw, c.tile_order, stream);
+    case 1: return batch_inline_nt<1>(c.nontemporal, alpha, grid, block, wl, ntex, ntiles, tpw, c.tile_order, stream);
+    case 2: return batch_inline_nt<2>(c.nontemporal, alpha, grid, block, wl, ntex, ntiles, tpw, c.tile_order, stream);
+    default: return batch_inline_nt<4>(c.nontemporal, alpha, grid, block, wl, ntex, ntiles, tpw, c.tile_order, stream);
     }
 }
 

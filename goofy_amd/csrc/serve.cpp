@@ -199,6 +199,7 @@ bool goofy_server::launch_group(Slot& s, std::vector<Req>& group)
         }
         const uint64_t tile = (uint64_t)tile_blocks(L);
         uint32_t tiles = 0;
+        bool alpha = false;  // the group holds DXT5 / ETC2 RGBA8 requests
         for (size_t i = 0; i < group.size(); ++i) {
             const goofy_request& r = group[i].r;
             TexDesc d{};
@@ -209,6 +210,7 @@ bool goofy_server::launch_group(Slot& s, std::vector<Req>& group)
             d.nblocks = d.bw * (r.height / 4);
             d.magic = div_magic(d.bw);
             d.fmt = (uint32_t)r.format;
+            alpha |= r.format >= GOOFY_FORMAT_DXT5;
             d.tile_begin = tiles;
             tiles += (uint32_t)((d.nblocks + tile - 1) / tile);
             descs[i] = d;
@@ -216,10 +218,10 @@ bool goofy_server::launch_group(Slot& s, std::vector<Req>& group)
         }
         begins[group.size()] = tiles;
         if (inline_list) {
-            e = launch_batch_inline(L, wl, (uint32_t)group.size(), tiles, num_cus, s.stream);
+            e = launch_batch_inline(L, wl, (uint32_t)group.size(), tiles, num_cus, s.stream, alpha);
         } else {
             e = hipMemcpyAsync(s.d_desc, s.h_desc, worklist_bytes(group.size()), hipMemcpyHostToDevice, s.stream);
-            if (e == hipSuccess) e = launch_batch(L, s.d_desc, (uint32_t)group.size(), tiles, num_cus, s.stream);
+            if (e == hipSuccess) e = launch_batch(L, s.d_desc, (uint32_t)group.size(), tiles, num_cus, s.stream, alpha);
         }
     } else {
         for (Req& q : group) {
@@ -402,7 +404,7 @@ int goofy_server_create(const goofy_server_config* ucfg, goofy_server** out)
 
 static int validate_request(const goofy_server* s, const goofy_request* r, Req* q)
 {
-    if (r->format != GOOFY_FORMAT_DXT1 && r->format != GOOFY_FORMAT_ETC1) return fail(GOOFY_ERR_INVALID, "unknown format");
+    if (!format_valid(r->format)) return fail(GOOFY_ERR_INVALID, "unknown format");
     if (r->width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
     if (r->height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
     if (r->width == 0 || r->height == 0) return fail(GOOFY_ERR_INVALID, "empty texture");
@@ -413,7 +415,7 @@ static int validate_request(const goofy_server* s, const goofy_request* r, Req* 
     if (!magic_ok(bw, nb) || nb > 0xFFFFFFFFull) return fail(GOOFY_ERR_TOO_LARGE, "texture too large");
     q->r = *r;
     q->in_bytes = r->pitch * r->height;
-    q->out_bytes = nb * 8;
+    q->out_bytes = nb * block_bytes(r->format);
     if (q->in_bytes + align_up(q->out_bytes) > s->cfg.hbm_budget_bytes || align_up(q->out_bytes) > s->arena.size())
         return fail(GOOFY_ERR_INVALID, "request larger than the server's budget or arena");
     return GOOFY_OK;

@@ -159,7 +159,7 @@ def make_pool(g, hip, textures: list[dict], stream=None):
     sizes = [(t["width"] * 4 * t["height"] + 255) // 256 * 256 for t in textures]
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     src = hip.DeviceBuffer(int(offs[-1]))
-    osz = [(t["width"] // 4) * (t["height"] // 4) * 8 for t in textures]
+    osz = [(t["width"] // 4) * (t["height"] // 4) * g.block_bytes(t["format"]) for t in textures]
     ooffs = np.concatenate([[0], np.cumsum([(s + 255) // 256 * 256 for s in osz])]).astype(np.int64)
     alone = hip.DeviceBuffer(int(ooffs[-1]))
     pool = []

@@ -29,7 +29,7 @@ extern "C" {
  * level (BC3: a0 = max, a1 = min alpha; EAC: table 9, multiplier
  * max(1, ceil((max-min)/19)), base on the minimum or, near 255, the maximum).
  * The host drop-in, the device entry points, the decoders, quality sums and
- * containers take all four; batches and the server take DXT1 and ETC1S. */
+ * containers, batches and the server take all four. */
 typedef enum goofy_format {
     GOOFY_FORMAT_DXT1 = 0,
     GOOFY_FORMAT_ETC1 = 1,

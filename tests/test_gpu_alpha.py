@@ -113,3 +113,67 @@ def test_alpha_dropin_vs_oracle(gpu, oracle, w, h, extra):
         assert rc == 0 and np.array_equal(out, want)
         assert np.array_equal(gpu.compress(fmt, img, w, h, stride), want)
         assert fn(out, img, w - 4, h, stride) == -1 and fn(out, img, w, h - 2, stride) == -2
+
+
+def mixed_textures(n, seed, lo, hi):
+    """Textures of all four formats (alpha content), sizes log-uniform multiples of 4."""
+    rng = np.random.default_rng(seed)
+    ts = W.batch_textures(n=n, lo=lo, hi=hi, seed=seed)
+    fmts = rng.integers(0, 4, n)
+    return [dict(t, kind=W.WAVES_ALPHA, format=int(f)) for t, f in zip(ts, fmts)]
+
+
+def test_alpha_batch_all_formats(gpu, oracle):
+    from goofy_amd import hip
+    ts = mixed_textures(40, 78, 4, 600)
+    items, want = [], []
+    for t in ts:
+        src = dev_buffer(t["width"] * 4 * t["height"])
+        gpu.synth_device(t["kind"], t["seed"], t["texture_id"], t["noise"], src, t["width"], t["height"])
+        n = (t["width"] // 4) * (t["height"] // 4) * gpu.block_bytes(t["format"])
+        items.append(dict(src=src, dst=dev_buffer(n, fill=0xCD), width=t["width"], height=t["height"],
+                          pitch=t["width"] * 4, format=t["format"]))
+        img = gpu.synth_host(t["kind"], t["seed"], t["texture_id"], t["noise"], t["width"], t["height"])
+        want.append(oracle.compress_blocks(t["format"], img, t["width"], t["height"], t["width"] * 4))
+    assert {t["format"] for t in ts} == {0, 1, 2, 3}
+    b = gpu.Batch(items)
+
+    def check(tag):
+        for it, wv in zip(items, want):
+            assert np.array_equal(to_host(it["dst"]), wv), (tag, it["format"], it["width"], it["height"])
+
+    def clear():
+        for it in items:
+            it["dst"].fill(0xCD)
+
+    say(f"alpha batch 40 textures, all formats, config={b.config}")
+    b.launch()
+    check("graph")
+    for r in range(2):
+        clear()
+        hip.synchronize()
+        b.launch()
+        check(f"replay {r}")
+    clear()
+    b.launch_per_texture()
+    check("per-texture")
+    for cfg in (dict(blocks_per_lane=2, wg_waves=4, waves_per_cu=0, nontemporal=1, tile_order=1),
+                dict(blocks_per_lane=4, wg_waves=8, waves_per_cu=2, nontemporal=0, tile_order=0)):
+        clear()
+        b.config = cfg
+        say(f"alpha batch config {cfg}")
+        b.launch()
+        check(str(cfg))
+    b.close()
+
+
+def test_alpha_serving_all_formats(gpu):
+    from goofy_amd import hip, serving
+    ts = mixed_textures(32, 79, 4, 640)
+    pool, expected, keep = serving.make_pool(gpu, hip, ts)
+    say("alpha serving: Poisson 20k req/s, 0.3 s, every result verified")
+    res = serving.simulate(pool, 20000.0, 0.3, seed=4, expected=expected, verify_every=1)
+    say(res)
+    keep.free()
+    assert res["completed"] == res["submitted"] > 2000
+    assert res["verified"] == res["completed"] and res["mismatches"] == 0
