@@ -480,9 +480,10 @@ def profile_mode(g, hip, W, launches: int):
     """Short run for rocprofv3, all on one stream: each plain kernel `launches`
     times (default launch shape), the traffic probe `launches` times per
     workgroup size (1, 2, 4, 8 waves), the dual-format kernel `launches` times,
-    and a device-to-device copy of the 256 MiB image (the runtime's copy
-    kernel) used to calibrate the HBM byte counters.  Inputs rotate over four
-    256 MiB buffers."""
+    the DXT5 / ETC2 RGBA8 kernels `launches` times each on the image with an
+    alpha channel, and a device-to-device copy of the 256 MiB image (the
+    runtime's copy kernel) used to calibrate the HBM byte counters.  Inputs
+    rotate over four 256 MiB buffers."""
     im = W.IMAGE_8192
     nbytes = im.pitch * im.height
     srcs = [hip.DeviceBuffer(nbytes) for _ in range(4)]
@@ -502,6 +503,14 @@ def profile_mode(g, hip, W, launches: int):
     out2 = hip.DeviceBuffer(im.blocks * 8)
     for i in range(launches):
         g.compress_device_dual(out, out2, srcs[i % 4], im.width, im.height, im.pitch)
+    hip.synchronize()
+    ia = W.ALPHA_8192
+    for s in srcs:
+        g.synth_device(ia.kind, ia.seed, ia.texture_id, ia.noise, s, ia.width, ia.height, ia.pitch)
+    out16 = hip.DeviceBuffer(ia.blocks * 16)
+    for f in (W.DXT5, W.ETC2):
+        for i in range(launches):
+            g.compress_device(f, out16, srcs[i % 4], ia.width, ia.height, ia.pitch)
     hip.synchronize()
     log(f"profile mode done: {launches} launches per kernel")
 
