@@ -337,7 +337,9 @@ def bench_around(g, hip, W, reps: int = 20):
     (decoders, floor: one launch each) or around the whole call (quality:
     zeroing kernel, sum kernel, read-back), and GB/s of algorithmic bytes --
     decode 8 B in + 64 B out per block, floor 4 B in + 4 B out per pixel,
-    round-trip quality 8 + 64 B read per block, image quality 8 B read per pixel."""
+    round-trip quality 8 + 64 B read per block, image quality 8 B read per pixel.
+    The DXT5 / ETC2 RGBA8 decoders and round trips run on the image with an
+    alpha channel (16 B blocks: 80 B per block)."""
     im = W.IMAGE_8192
     stream = hip.Stream()
     src = hip.DeviceBuffer(im.pitch * im.height)
@@ -370,6 +372,21 @@ def bench_around(g, hip, W, reps: int = 20):
         ms = timed(lambda: q.update(g.quality_device(f, blocks, src, im.width, im.height, im.pitch, stream)), 5)
         res[f"quality_{key}"] = {**entry(ms, im.blocks * 72), "psnr_rgb": round(q["psnr_rgb"], 3),
                                  "psnr_y": round(q["psnr_y"], 3)}
+    ia = W.ALPHA_8192
+    asrc = hip.DeviceBuffer(ia.pitch * ia.height)
+    g.synth_device(ia.kind, ia.seed, ia.texture_id, ia.noise, asrc, ia.width, ia.height, ia.pitch, stream)
+    blocks16 = hip.DeviceBuffer(ia.blocks * 16)
+    for f, key in ((W.DXT5, "dxt5"), (W.ETC2, "etc2")):
+        g.compress_device(f, blocks16, asrc, ia.width, ia.height, ia.pitch, stream)
+        g.decode_device(f, img, blocks16, ia.width, ia.height, ia.pitch, stream)
+        res[f"decode_{key}"] = entry(timed(lambda: g.decode_device(f, img, blocks16, ia.width, ia.height, ia.pitch,
+                                                                   stream)), ia.blocks * 80)
+        q = {}
+        ms = timed(lambda: q.update(g.quality_device(f, blocks16, asrc, ia.width, ia.height, ia.pitch, stream)), 5)
+        res[f"quality_{key}"] = {**entry(ms, ia.blocks * 80), "psnr_rgb": round(q["psnr_rgb"], 3),
+                                 "psnr_a": round(q["psnr_a"], 3)}
+    asrc.free()
+    blocks16.free()
     g.rgb565_baseline_device(img, src, im.width, im.height, im.pitch, im.pitch, stream)
     res["rgb565_floor"] = entry(timed(lambda: g.rgb565_baseline_device(img, src, im.width, im.height, im.pitch,
                                                                        im.pitch, stream)), im.pixels * 8)
