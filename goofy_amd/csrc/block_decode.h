@@ -20,48 +20,57 @@
 
 namespace goofy_dev {
 
-__device__ __forceinline__ uint32_t pack_rgba(uint32_t r, uint32_t g, uint32_t b, uint32_t a)
-{
-    return r | (g << 8) | (b << 16) | (a << 24);
-}
-
 // 16 decoded pixels (row-major) of a BC1 block; FOUR: the colour block of
-// BC3, always in 4-colour mode.
+// BC3, always in 4-colour mode.  Per channel the 4 palette bytes sit in one
+// word (byte k = entry k); a row's pixels come from one v_perm_b32 per
+// channel whose selector bytes are the pixels' 2-bit indices, and four more
+// interleave the channel rows into RGBA words.
 template <bool FOUR = false>
 __device__ __forceinline__ void decode_bc1(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
 {
     const uint32_t c0 = w0 & 0xFFFFu, c1 = w0 >> 16;
     const uint32_t r0 = (c0 >> 11) & 31u, g0 = (c0 >> 5) & 63u, b0 = c0 & 31u;
     const uint32_t r1 = (c1 >> 11) & 31u, g1 = (c1 >> 5) & 63u, b1 = c1 & 31u;
-    const uint32_t R0 = (r0 << 3) | (r0 >> 2), G0 = (g0 << 2) | (g0 >> 4), B0 = (b0 << 3) | (b0 >> 2);
-    const uint32_t R1 = (r1 << 3) | (r1 >> 2), G1 = (g1 << 2) | (g1 >> 4), B1 = (b1 << 3) | (b1 >> 2);
-    uint32_t pal[4];
-    pal[0] = pack_rgba(R0, G0, B0, 255u);
-    pal[1] = pack_rgba(R1, G1, B1, 255u);
-    if (FOUR || c0 > c1) {
-        pal[2] = pack_rgba((2u * R0 + R1) / 3u, (2u * G0 + G1) / 3u, (2u * B0 + B1) / 3u, 255u);
-        pal[3] = pack_rgba((R0 + 2u * R1) / 3u, (G0 + 2u * G1) / 3u, (B0 + 2u * B1) / 3u, 255u);
-    } else {
-        pal[2] = pack_rgba((R0 + R1) / 2u, (G0 + G1) / 2u, (B0 + B1) / 2u, 255u);
-        pal[3] = 0u;
-    }
+    const uint32_t e0[3] = {(r0 << 3) | (r0 >> 2), (g0 << 2) | (g0 >> 4), (b0 << 3) | (b0 >> 2)};
+    const uint32_t e1[3] = {(r1 << 3) | (r1 >> 2), (g1 << 2) | (g1 >> 4), (b1 << 3) | (b1 >> 2)};
+    const bool four = FOUR || c0 > c1;
+    uint32_t plane[3];
 #pragma unroll
-    for (int p = 0; p < 16; ++p) {
-        const uint32_t k = (w1 >> (2 * p)) & 3u;
-        const uint32_t lo = (k & 1u) ? pal[1] : pal[0];
-        const uint32_t hi = (k & 1u) ? pal[3] : pal[2];
-        px[p] = (k & 2u) ? hi : lo;
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t e2 = four ? (2u * e0[c] + e1[c]) / 3u : (e0[c] + e1[c]) / 2u;
+        const uint32_t e3 = four ? (e0[c] + 2u * e1[c]) / 3u : 0u;
+        plane[c] = e0[c] | (e1[c] << 8) | (e2 << 16) | (e3 << 24);
     }
-}
-
-__device__ __forceinline__ uint32_t clamp_add(uint32_t c, int32_t d)
-{
-    const int32_t v = (int32_t)c + d;
-    return (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+    const uint32_t alpha = four ? 0xFFFFFFFFu : 0x00FFFFFFu;  // 3-colour mode: entry 3 is transparent black
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        const uint32_t x = (w1 >> (8 * y)) & 0xFFu;  // pixel c's index at bits 2c
+        const uint32_t t = (x & 0x0Fu) | ((x & 0xF0u) << 12);
+        const uint32_t sel = (t & 0x00030003u) | ((t & 0x000C000Cu) << 6);  // pixel c's index in byte c
+        const uint32_t r = __builtin_amdgcn_perm(0u, plane[0], sel);
+        const uint32_t g = __builtin_amdgcn_perm(0u, plane[1], sel);
+        const uint32_t bl = __builtin_amdgcn_perm(0u, plane[2], sel);
+        const uint32_t a = __builtin_amdgcn_perm(0u, alpha, sel);
+        const uint32_t rg01 = __builtin_amdgcn_perm(g, r, 0x05010400u);   // R0 G0 R1 G1
+        const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);   // R2 G2 R3 G3
+        const uint32_t ba01 = __builtin_amdgcn_perm(a, bl, 0x05010400u);  // B0 A0 B1 A1
+        const uint32_t ba23 = __builtin_amdgcn_perm(a, bl, 0x07030602u);  // B2 A2 B3 A3
+        px[4 * y + 0] = __builtin_amdgcn_perm(ba01, rg01, 0x05040100u);
+        px[4 * y + 1] = __builtin_amdgcn_perm(ba01, rg01, 0x07060302u);
+        px[4 * y + 2] = __builtin_amdgcn_perm(ba23, rg23, 0x05040100u);
+        px[4 * y + 3] = __builtin_amdgcn_perm(ba23, rg23, 0x07060302u);
+    }
 }
 
 // 16 decoded pixels (row-major) of an ETC1 block; w0/w1 are the block's
 // first/second 4 bytes read as little-endian words.
+//
+// Per channel, the 8 clamped palette bytes of both sub-blocks sit in two
+// words (sub-block 0 in the low word, entry = selector value).  A row's four
+// pixels then come from one v_perm_b32 per channel whose selector bytes are
+// the pixels' 2-bit selectors plus 4 for sub-block 1, and four more
+// v_perm_b32 interleave the channel rows into RGBA words (alpha 255 from
+// selector byte 0x0d).
 __device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
 {
     const uint32_t hi = __builtin_bswap32(w0), lo = __builtin_bswap32(w1);  // big-endian words
@@ -82,32 +91,43 @@ __device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (
             base[1][c] = (b4 << 4) | b4;
         }
     }
-    const uint32_t cw[2] = {(hi >> 5) & 7u, (hi >> 2) & 7u};
-    uint32_t pal[2][4];
+    // plane[c][s]: byte k = clamp(base[s][c] + modifier k), k = selector value
+    uint32_t plane[3][2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
+        const uint32_t cw = (hi >> (5 - 3 * s)) & 7u;
         // ETC1 intensity modifiers {small, large} per table index, one byte each:
         // {2,8},{5,17},{9,29},{13,42},{18,60},{24,80},{33,106},{47,183}
-        const uint32_t a = (uint32_t)((0x2F2118120D090502ull >> (8 * cw[s])) & 0xFFu);
-        const uint32_t b = (uint32_t)((0xB76A503C2A1D1108ull >> (8 * cw[s])) & 0xFFu);
-        const int32_t mods[4] = {(int32_t)a, (int32_t)b, -(int32_t)a, -(int32_t)b};  // selector 0..3
+        const int32_t a = (int32_t)((0x2F2118120D090502ull >> (8 * cw)) & 0xFFu);
+        const int32_t b = (int32_t)((0xB76A503C2A1D1108ull >> (8 * cw)) & 0xFFu);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            pal[s][k] = pack_rgba(clamp_add(base[s][0], mods[k]), clamp_add(base[s][1], mods[k]),
-                                  clamp_add(base[s][2], mods[k]), 255u);
-    }
-#pragma unroll
-    for (int y = 0; y < 4; ++y)
-#pragma unroll
-        for (int x = 0; x < 4; ++x) {
-            const int k = 4 * x + y;
-            const uint32_t sel = (((lo >> (16 + k)) & 1u) << 1) | ((lo >> k) & 1u);
-            const bool second = flip ? (y >= 2) : (x >= 2);
-            const uint32_t* P = second ? pal[1] : pal[0];
-            const uint32_t l = (sel & 1u) ? P[1] : P[0];
-            const uint32_t h = (sel & 1u) ? P[3] : P[2];
-            px[4 * y + x] = (sel & 2u) ? h : l;
+        for (int c = 0; c < 3; ++c) {
+            const int32_t v = (int32_t)base[s][c];
+            plane[c][s] = (uint32_t)min(max(v + a, 0), 255) | ((uint32_t)min(max(v + b, 0), 255) << 8) |
+                          ((uint32_t)min(max(v - a, 0), 255) << 16) | ((uint32_t)min(max(v - b, 0), 255) << 24);
         }
+    }
+    // sub-block 1: rows 2..3 (flip) or columns 2..3
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        // selector bits of pixel (x, y) at k = 4x + y: LSB plane bit k, MSB plane bit 16 + k
+        const uint32_t l = (lo >> y) & 0x1111u, m = (lo >> (16 + y)) & 0x1111u;
+        const uint32_t v = l | (m << 1);  // pixel x's selector in bits 4x..4x+1
+        const uint32_t w = (v & 0xFFu) | ((v & 0xFF00u) << 8);
+        uint32_t sel = (w & 0x000F000Fu) | ((w & 0x00F000F0u) << 4);  // pixel x's selector in byte x
+        sel |= flip ? (y >= 2 ? 0x04040404u : 0u) : 0x04040000u;
+        const uint32_t r = __builtin_amdgcn_perm(plane[0][1], plane[0][0], sel);
+        const uint32_t g = __builtin_amdgcn_perm(plane[1][1], plane[1][0], sel);
+        const uint32_t bl = __builtin_amdgcn_perm(plane[2][1], plane[2][0], sel);
+        const uint32_t rg01 = __builtin_amdgcn_perm(g, r, 0x05010400u);   // R0 G0 R1 G1
+        const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);   // R2 G2 R3 G3
+        const uint32_t ba01 = __builtin_amdgcn_perm(0u, bl, 0x0d010d00u); // B0 FF B1 FF
+        const uint32_t ba23 = __builtin_amdgcn_perm(0u, bl, 0x0d030d02u); // B2 FF B3 FF
+        px[4 * y + 0] = __builtin_amdgcn_perm(ba01, rg01, 0x05040100u);
+        px[4 * y + 1] = __builtin_amdgcn_perm(ba01, rg01, 0x07060302u);
+        px[4 * y + 2] = __builtin_amdgcn_perm(ba23, rg23, 0x05040100u);
+        px[4 * y + 3] = __builtin_amdgcn_perm(ba23, rg23, 0x07060302u);
+    }
 }
 
 // Alpha of four pixels from an 8-entry palette held as bytes of {hi:lo}:

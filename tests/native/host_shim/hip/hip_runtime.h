@@ -25,15 +25,18 @@ using std::max;
 using std::min;
 
 // V_PERM_B32: byte i of the result is byte sel_i of the 8-byte value
-// {hi:lo} (0..3 from lo, 4..7 from hi); selector 0x0c gives 0x00.  (Other
-// selector values -- sign replication, 0xff -- are not used by the codec.)
+// {hi:lo} (0..3 from lo, 4..7 from hi); selectors 8..11 replicate the sign
+// bit of byte 1, 3, 5, 7; 12 gives 0x00 and 13 and above 0xFF.
 inline uint32_t goofy_host_perm(uint32_t hi, uint32_t lo, uint32_t sel)
 {
     const uint64_t v = ((uint64_t)hi << 32) | lo;
     uint32_t out = 0;
     for (int i = 0; i < 4; ++i) {
         const uint32_t s = (sel >> (8 * i)) & 0xFFu;
-        const uint32_t b = s < 8 ? (uint32_t)(v >> (8 * s)) & 0xFFu : (s == 0x0c ? 0u : 0xFFu);
+        uint32_t b;
+        if (s < 8) b = (uint32_t)(v >> (8 * s)) & 0xFFu;
+        else if (s < 12) b = ((v >> (8 * (2 * (s - 8) + 1) + 7)) & 1u) ? 0xFFu : 0u;
+        else b = s == 12 ? 0u : 0xFFu;
         out |= b << (8 * i);
     }
     return out;
