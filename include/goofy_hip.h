@@ -65,7 +65,8 @@ GOOFY_HIP_API int goofy_hip_device_count(void);
 /* ---- drop-in, host memory ----------------------------------------------
  * Same contract as goofy::compressDXT1/compressETC1 (GoofyTC/goofy_tc.h:1497-1557):
  * host `input` RGBA8 with `stride` bytes per row, host `result` receives
- * (width/4)*(height/4)*8 tightly packed bytes.  -1 if width % 16, then -2 if
+ * (width/4)*(height/4) tightly packed blocks (8 bytes each; 16 for DXT5 and
+ * ETC2 RGBA8).  -1 if width % 16, then -2 if
  * height % 4; width or height 0 writes nothing and returns 0.  Synchronous:
  * stages through device memory on a per-thread stream and returns when
  * `result` holds the blocks.  Thread-safe. */
@@ -98,8 +99,9 @@ GOOFY_HIP_API int goofy_hip_compress_dual(unsigned char* result_dxt1, unsigned c
  * `stream` (a hipStream_t, NULL = default stream) and returns after the
  * launch.  Per-block semantics: width and height multiples of 4 (width % 16
  * is NOT required), `pitch` >= 4*width and a multiple of 16, `d_input`
- * 16-byte aligned, `d_result` 8-byte aligned.  Output is bit-identical to the
- * reference on every block. */
+ * 16-byte aligned, `d_result` aligned to the block size (8 or 16 bytes).
+ * Output is bit-identical to the reference on every block (for DXT5 / ETC2
+ * RGBA8: the colour half; the alpha half follows the rule above). */
 GOOFY_HIP_API int goofy_hip_compress_device(int format, void* d_result, const void* d_input, unsigned int width,
                                             unsigned int height, size_t pitch, void* stream);
 
@@ -224,7 +226,7 @@ GOOFY_HIP_API int goofy_hip_traffic_probe(void* d_out, const void* d_input, unsi
  * them with one persistent kernel launch, captured once into a hipGraph. */
 typedef struct goofy_texture_desc {
     const void* src;     /* device RGBA8, 16-byte aligned */
-    void* dst;           /* device, (width/4)*(height/4)*8 bytes */
+    void* dst;           /* device, (width/4)*(height/4) blocks, aligned to the block size */
     unsigned int width;  /* multiple of 4 */
     unsigned int height; /* multiple of 4 */
     size_t pitch;        /* bytes between rows: >= 4*width, multiple of 16 */
