@@ -514,3 +514,22 @@ def test_dropin_dual_return_codes(gpu):
     assert f(gpu._u8(out_a), gpu._u8(out_b), gpu._u8(img), 64, 62, 256) == -2
     assert f(gpu._u8(out_a), gpu._u8(out_b), gpu._u8(img), 0, 8, 0) == 0
     assert f(gpu._u8(out_a), gpu._u8(None), gpu._u8(img), 64, 64, 256) == gpu.ERR_INVALID
+
+
+@pytest.mark.parametrize("w,h,extra", [(2048, 8192, 0), (4096, 4100, 48)])
+def test_dropin_many_chunks_matches_device(gpu, w, h, extra):
+    """Host images of 8 and 16 staging chunks (every slot reused several times,
+    the last chunk short): the drop-in against the device entry point."""
+    stride = w * 4 + extra
+    img = gpu.synth_host(0, 4242, 1, 20, w, h, stride, fill=0x11)
+    src = upload(img)
+    n = (w // 4) * (h // 4) * 8
+    for fmt, fn in ((W.DXT1, gpu.compressDXT1), (W.ETC1, gpu.compressETC1)):
+        say(f"drop-in many chunks fmt={fmt} {w}x{h} stride={stride}")
+        want = encode_dev(gpu, fmt, src, w, h, stride)
+        out = np.full(n, 0xCD, np.uint8)
+        assert fn(out, img, w, h, stride) == 0
+        assert np.array_equal(out, want)
+    a, b = gpu.compress_dual(img, w, h, stride)
+    assert np.array_equal(a, encode_dev(gpu, W.DXT1, src, w, h, stride))
+    assert np.array_equal(b, encode_dev(gpu, W.ETC1, src, w, h, stride))
