@@ -109,9 +109,11 @@ bool cfg_valid(const LaunchCfg& c)
 // k-2.  Uploads go on a stream of their own, back to back: the kernel and the
 // download of a chunk wait for its upload on the other stream, and a slot is
 // refilled only after the host has collected its previous chunk, so the
-// upload stream never waits on the device.  (Measured with a memory-copy
-// trace: on one stream each upload waited ~40 us behind the previous chunk's
-// kernel and download.)  A one-chunk image stays on one stream.  Row copies of large chunks are split over a small persistent thread
+// upload stream never waits on the device.  (On a single stream each upload
+// is ordered behind the previous chunk's kernel and download; on MI355X the
+// 8192^2 call took 6.6 ms that way and 5.5 ms with the uploads apart, at the
+// link's measured host-to-device ceiling.)  A one-chunk image stays on one
+// stream.  Row copies of large chunks are split over a small persistent thread
 // pool.  Contexts (stream, events, device and pinned buffers) come from a
 // process-wide pool: concurrent callers never share one, and nothing is freed
 // at thread exit (the HIP runtime may already be gone then).
