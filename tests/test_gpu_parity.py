@@ -533,3 +533,36 @@ def test_dropin_many_chunks_matches_device(gpu, w, h, extra):
     a, b = gpu.compress_dual(img, w, h, stride)
     assert np.array_equal(a, encode_dev(gpu, W.DXT1, src, w, h, stride))
     assert np.array_equal(b, encode_dev(gpu, W.ETC1, src, w, h, stride))
+
+
+def test_dropin_from_threads(gpu):
+    """The drop-in on four host threads at once, each with its own images of
+    one to several staging chunks: every call bit-identical to the same call
+    made alone."""
+    import threading
+    shapes = [(512, 512), (1024, 2052), (2048, 3072), (4096, 1028)]
+    imgs = [gpu.synth_host(0, 77 + i, i, 15, w, h) for i, (w, h) in enumerate(shapes)]
+    fmts = [W.DXT1, W.ETC1, W.DXT5, W.ETC2]
+    alone = {}
+    for i, (w, h) in enumerate(shapes):
+        for f in fmts:
+            alone[i, f] = gpu.compress(f, imgs[i], w, h)
+    errors = []
+
+    def worker(i):
+        w, h = shapes[i]
+        try:
+            for rep in range(6):
+                for f in fmts:
+                    if not np.array_equal(gpu.compress(f, imgs[i], w, h), alone[i, f]):
+                        errors.append((i, f, rep))
+        except Exception as e:  # reported below, on the main thread
+            errors.append(repr(e))
+
+    say("drop-in: 4 threads x 24 calls")
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors[:5]
