@@ -117,7 +117,10 @@ def test_device_api_validation_without_device():
                  (goofy_amd.DXT1, 8, 16, 8, 4, 16),      # pitch < 4*width
                  (goofy_amd.DXT1, 8, 20, 8, 4, 32),      # misaligned input
                  (goofy_amd.DXT1, 0, 16, 8, 4, 32),      # null output
-                 (7, 8, 16, 8, 4, 32)]:                  # unknown format
+                 (goofy_amd.DXT5, 8, 16, 8, 4, 32),      # 16-byte blocks need a 16-byte aligned output
+                 (goofy_amd.ETC2, 24, 16, 8, 4, 32),
+                 (4, 16, 16, 8, 4, 32),                  # unknown format
+                 (7, 8, 16, 8, 4, 32)]:
         with pytest.raises(goofy_amd.GoofyError) as e:
             goofy_amd.compress_device(*args)
         assert e.value.status == goofy_amd.ERR_INVALID, args
@@ -192,6 +195,11 @@ def test_batch_validation_without_device():
     assert e.value.status == goofy_amd.ERR_WIDTH
     bad = [dict(src=16, dst=8, width=8, height=4, pitch=32, format=0),
            dict(src=16, dst=8, width=8, height=4, pitch=40, format=0)]
+    with pytest.raises(goofy_amd.GoofyError) as e:
+        goofy_amd.Batch(bad)
+    assert e.value.status == goofy_amd.ERR_INVALID and "texture 1" in goofy_amd.last_error()
+    bad = [dict(src=16, dst=16, width=8, height=4, pitch=32, format=goofy_amd.DXT5),
+           dict(src=16, dst=8, width=8, height=4, pitch=32, format=goofy_amd.ETC2)]  # 16-byte blocks, 8-aligned
     with pytest.raises(goofy_amd.GoofyError) as e:
         goofy_amd.Batch(bad)
     assert e.value.status == goofy_amd.ERR_INVALID and "texture 1" in goofy_amd.last_error()
