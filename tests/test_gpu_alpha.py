@@ -177,3 +177,19 @@ def test_alpha_serving_all_formats(gpu):
     keep.free()
     assert res["completed"] == res["submitted"] > 2000
     assert res["verified"] == res["completed"] and res["mismatches"] == 0
+
+
+def test_alpha_large_fuzz_vs_oracle(gpu, oracle):
+    """4,194,304 random blocks (alpha kind: six alpha distributions per block,
+    random colour) in one 8192 x 2048 image, both 16-byte formats against the
+    multi-threaded oracle."""
+    from blockgen import block_image
+    bw, bh = 2048, 512
+    img = block_image("alpha", 2718, bw, bh)
+    src = upload(img)
+    for fmt in (W.DXT5, W.ETC2):
+        say(f"alpha large fuzz fmt={fmt}: {bw * bh} blocks")
+        got = encode_dev(gpu, fmt, src, 4 * bw, 4 * bh, 16 * bw)
+        want = oracle.compress_blocks(fmt, img, 4 * bw, 4 * bh, 16 * bw)
+        bad = np.nonzero((got.reshape(-1, 16) != want.reshape(-1, 16)).any(1))[0]
+        assert bad.size == 0, f"{bad.size} blocks differ, first {bad[:8]}"
