@@ -823,4 +823,14 @@ __attribute__((visibility("default"))) int compressETC1(unsigned char* result, c
 {
     return goofy_impl::compress_host(GOOFY_FORMAT_ETC1, result, input, width, height, stride);
 }
+__attribute__((visibility("default"))) int compressDXT5(unsigned char* result, const unsigned char* input,
+                                                        unsigned int width, unsigned int height, unsigned int stride)
+{
+    return goofy_impl::compress_host(GOOFY_FORMAT_DXT5, result, input, width, height, stride);
+}
+__attribute__((visibility("default"))) int compressETC2(unsigned char* result, const unsigned char* input,
+                                                        unsigned int width, unsigned int height, unsigned int stride)
+{
+    return goofy_impl::compress_host(GOOFY_FORMAT_ETC2, result, input, width, height, stride);
+}
 }  // namespace goofy

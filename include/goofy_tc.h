@@ -24,6 +24,16 @@ int compressDXT1(unsigned char* result, const unsigned char* input, unsigned int
 // Replaces goofy::compressETC1, GoofyTC/goofy_tc.h:12 (definition :1528-1557).
 int compressETC1(unsigned char* result, const unsigned char* input, unsigned int width, unsigned int height,
                  unsigned int stride);
+
+// No counterpart in the reference, whose README names a DXT5 / ETC2 alpha
+// encoder as its next step: the same contract with 16-byte blocks, i.e.
+// `result` holds (width/4)*(height/4)*16 bytes.  DXT5 is BC3 and ETC2 is ETC2
+// RGBA8 (EAC alpha); each block is an alpha block followed by the block
+// compressDXT1 / compressETC1 write for the same pixels.
+int compressDXT5(unsigned char* result, const unsigned char* input, unsigned int width, unsigned int height,
+                 unsigned int stride);
+int compressETC2(unsigned char* result, const unsigned char* input, unsigned int width, unsigned int height,
+                 unsigned int stride);
 }  // namespace goofy
 
 #endif
