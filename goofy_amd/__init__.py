@@ -195,6 +195,19 @@ def compress(fmt: int, image: np.ndarray, width: int, height: int, stride: int |
 _sig("goofy_hip_compress_dual", ctypes.c_int, _u8p, _u8p, _u8p, _uint, _uint, _uint)
 
 
+_sig("goofy_hip_compress_dual_alpha", ctypes.c_int, _u8p, _u8p, _u8p, _uint, _uint, _uint)
+
+
+def compress_dual_alpha(image: np.ndarray, width: int, height: int, stride: int | None = None) -> tuple:
+    """DXT5 and ETC2 RGBA8 of a host RGBA8 image from one transfer and one pass: (dxt5, etc2) block bytes."""
+    stride = width * 4 if stride is None else stride
+    n = (width // 4) * (height // 4) * 16
+    a = np.empty(max(1, n), dtype=np.uint8)
+    b = np.empty(max(1, n), dtype=np.uint8)
+    _check(_lib.goofy_hip_compress_dual_alpha(_u8(a), _u8(b), _u8(image), width, height, stride))
+    return a[:n], b[:n]
+
+
 def compress_dual(image: np.ndarray, width: int, height: int, stride: int | None = None, out=None) -> tuple:
     """Both formats of a host RGBA8 image from one transfer and one pass: (dxt1, etc1s) block bytes.
     `out`: an optional pair of uint8 arrays of at least (w/4)*(h/4)*8 bytes each to write into."""
@@ -234,6 +247,17 @@ def compress_device_dual(dst_dxt1, dst_etc1, src, width: int, height: int, pitch
     _check(_lib.goofy_hip_compress_device_dual(_ptr(dst_dxt1), _ptr(dst_etc1), _ptr(src), width, height, pitch,
                                                _stream(stream), start_event.handle if start_event else None,
                                                stop_event.handle if stop_event else None))
+
+
+_sig("goofy_hip_compress_device_dual_alpha", ctypes.c_int, _vp, _vp, _vp, _uint, _uint, _sz, _vp, _vp, _vp)
+
+
+def compress_device_dual_alpha(dst_dxt5, dst_etc2, src, width: int, height: int, pitch: int, stream=None,
+                               start_event=None, stop_event=None) -> None:
+    """DXT5 and ETC2 RGBA8 of a device-resident image in one pass (one read of the image)."""
+    _check(_lib.goofy_hip_compress_device_dual_alpha(_ptr(dst_dxt5), _ptr(dst_etc2), _ptr(src), width, height, pitch,
+                                                     _stream(stream), start_event.handle if start_event else None,
+                                                     stop_event.handle if stop_event else None))
 
 
 _sig("goofy_hip_tune_device", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp, ctypes.c_int,

@@ -396,10 +396,9 @@ __device__ __forceinline__ uint32_t gather3(uint32_t i0, uint32_t i1, uint32_t i
     return (x & 0xC0C0C0C0u) | ((i0 >> 2) & 0x20202020u);
 }
 
-// BC3 alpha block as two little-endian words.
-__device__ __forceinline__ uint2 encode_bc3_alpha(const uint32_t (&p)[16])
+// BC3 alpha block as two little-endian words, from the block's alpha rows.
+__device__ __forceinline__ uint2 bc3_alpha(const AlphaRows& a)
 {
-    const AlphaRows a = alpha_rows(p);
     // level k = mn + floor(k*range/7); (x * 9363) >> 16 == x / 7 for x <= 1785
     const uint32_t range = a.mx - a.mn, r9 = range * 9363u, nmn = 255u - a.mn;
     uint32_t nlv[8];
@@ -430,10 +429,12 @@ __device__ __forceinline__ uint2 encode_bc3_alpha(const uint32_t (&p)[16])
     return make_uint2(a.mx | (a.mn << 8) | (lo << 16), (lo >> 16) | (hi << 16));
 }
 
-// EAC (ETC2 RGBA8 alpha) block as two little-endian words.
-__device__ __forceinline__ uint2 encode_eac_alpha(const uint32_t (&p)[16])
+__device__ __forceinline__ uint2 encode_bc3_alpha(const uint32_t (&p)[16]) { return bc3_alpha(alpha_rows(p)); }
+
+// EAC (ETC2 RGBA8 alpha) block as two little-endian words, from the block's
+// alpha rows.
+__device__ __forceinline__ uint2 eac_alpha(const AlphaRows& a)
 {
-    const AlphaRows a = alpha_rows(p);
     const uint32_t range = a.mx - a.mn;
     uint32_t m = ((range + 18u) * 3450u) >> 16;  // ceil(range / 19) for range <= 255
     m = m > 1u ? m : 1u;
@@ -466,6 +467,8 @@ __device__ __forceinline__ uint2 encode_eac_alpha(const uint32_t (&p)[16])
     const uint32_t head = (uint32_t)b | (((m << 4) | 9u) << 8);
     return make_uint2(perm(hi, head, 0x05060100u), perm(lo, hi, 0x04050600u));
 }
+
+__device__ __forceinline__ uint2 encode_eac_alpha(const uint32_t (&p)[16]) { return eac_alpha(alpha_rows(p)); }
 
 // DXT5 / ETC2 RGBA8 block: {alpha words, colour words}.
 template <int FMT>

@@ -225,14 +225,17 @@ static void copy_rows(uint8_t* dst, const uint8_t* src, size_t stride, size_t ro
 }
 
 // fmt: one format into `result`, or kHostDual: DXT1 into `result` and ETC1S
-// into `result2` from one pass over the image.
+// into `result2` from one pass over the image, or kHostDualAlpha: DXT5 and
+// ETC2 RGBA8 the same way.
 constexpr int kHostDual = 1 << 16;
+constexpr int kHostDualAlpha = (1 << 16) + 1;
 
 static int compress_host(int fmt, unsigned char* result, const unsigned char* input, unsigned width,
                          unsigned height, unsigned stride, unsigned char* result2 = nullptr)
 {
     clear_error();
-    const bool dual = fmt == kHostDual;
+    const bool dual_alpha = fmt == kHostDualAlpha;
+    const bool dual = fmt == kHostDual || dual_alpha;
     if (!format_valid(fmt) && !dual) return fail(GOOFY_ERR_INVALID, "unknown format");
     // Same checks, same order as the reference (GoofyTC/goofy_tc.h:1500-1508).
     if (width % 16 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 16");
@@ -256,9 +259,10 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
     const size_t row_bytes = (size_t)width * 4;
     const unsigned chunk_brows = (unsigned)std::max<size_t>(1, (size_t(8) << 20) / (row_bytes * 4));  // ~8 MiB
     const unsigned cb = std::min(std::min(chunk_brows, bh), max_rows);
-    // dual: a chunk's staging output holds its DXT1 blocks, then its ETC1S blocks
+    // dual: a chunk's staging output holds its blocks of the first format, then
+    // those of the second
     const size_t outs = dual ? 2 : 1;
-    const size_t bb = dual ? 8 : block_bytes(fmt);
+    const size_t bb = dual_alpha ? 16 : (dual ? 8 : block_bytes(fmt));
     if ((rc = ensure_ctx(ctx, row_bytes * 4 * cb, outs * bw * cb * bb)) != GOOFY_OK) return rc;
     const unsigned nchunks = (bh + cb - 1) / cb;
     const LaunchCfg cfg = auto_cfg((uint64_t)bw * cb, di.num_cus);
@@ -294,9 +298,12 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
             if ((e = hipStreamWaitEvent(ctx->stream, ctx->h2d_done[s], 0)) != hipSuccess) break;
         }
         const size_t n = (size_t)bw * rows * bb;
-        e = dual ? launch_dual((uint32_t)cfg.wg_waves, ctx->d_in[s], row_bytes, ctx->d_out[s],
-                               static_cast<uint8_t*>(ctx->d_out[s]) + n, bw, bw * rows, ctx->stream)
-                 : launch_plain(fmt, cfg, ctx->d_in[s], row_bytes, ctx->d_out[s], bw, bw * rows, ctx->stream);
+        uint8_t* second = static_cast<uint8_t*>(ctx->d_out[s]) + n;
+        e = dual_alpha ? launch_dual_alpha((uint32_t)cfg.wg_waves, ctx->d_in[s], row_bytes, ctx->d_out[s], second, bw,
+                                           bw * rows, ctx->stream)
+            : dual     ? launch_dual((uint32_t)cfg.wg_waves, ctx->d_in[s], row_bytes, ctx->d_out[s], second, bw,
+                                     bw * rows, ctx->stream)
+                       : launch_plain(fmt, cfg, ctx->d_in[s], row_bytes, ctx->d_out[s], bw, bw * rows, ctx->stream);
         if (e != hipSuccess) break;
         if ((e = hipMemcpyAsync(ctx->h_out[s], ctx->d_out[s], outs * n, hipMemcpyDeviceToHost, ctx->stream)) !=
             hipSuccess)
@@ -453,19 +460,19 @@ int goofy_hip_compress_dual(unsigned char* result_dxt1, unsigned char* result_et
     return compress_host(kHostDual, result_dxt1, input, width, height, stride, result_etc1);
 }
 
-int goofy_hip_compress_device_dual(void* d_dxt1, void* d_etc1, const void* d_input, unsigned int width,
-                                   unsigned int height, size_t pitch, void* stream, void* start_event,
-                                   void* stop_event)
+static int compress_device_pair(bool alpha, void* d_a, void* d_b, const void* d_input, unsigned int width,
+                                unsigned int height, size_t pitch, void* stream, void* start_event, void* stop_event)
 {
     clear_error();
     if (width % 4 != 0) return fail(GOOFY_ERR_WIDTH, "width must be a multiple of 4");
     if (height % 4 != 0) return fail(GOOFY_ERR_HEIGHT, "height must be a multiple of 4");
     if (width == 0 || height == 0) return GOOFY_OK;
-    if (!d_dxt1 || !d_etc1 || !d_input) return fail(GOOFY_ERR_INVALID, "null buffer");
+    if (!d_a || !d_b || !d_input) return fail(GOOFY_ERR_INVALID, "null buffer");
     if (pitch < (size_t)width * 4 || pitch % 16 != 0)
         return fail(GOOFY_ERR_INVALID, "pitch must be >= 4*width and a multiple of 16");
-    if ((uintptr_t)d_input % 16 != 0 || (uintptr_t)d_dxt1 % 8 != 0 || (uintptr_t)d_etc1 % 8 != 0)
-        return fail(GOOFY_ERR_INVALID, "input must be 16-byte and results 8-byte aligned");
+    const size_t bb = alpha ? 16 : 8;
+    if ((uintptr_t)d_input % 16 != 0 || (uintptr_t)d_a % bb != 0 || (uintptr_t)d_b % bb != 0)
+        return fail(GOOFY_ERR_INVALID, "input must be 16-byte and results block-size (8 or 16 byte) aligned");
     const uint32_t bw = width / 4, bh = height / 4;
     const uint32_t max_rows = max_stripe_brows(bw);
     if (max_rows == 0) return fail(GOOFY_ERR_TOO_LARGE, "image too wide (4,194,304 pixels or more per row)");
@@ -475,16 +482,41 @@ int goofy_hip_compress_device_dual(void* d_dxt1, void* d_etc1, const void* d_inp
     for (uint32_t y0 = 0; y0 < bh;) {  // stripes for images beyond one launch's index range
         const uint32_t rows = std::min(max_rows, bh - y0);
         const uint32_t nb = rows * bw;
-        const size_t out_off = (size_t)y0 * bw * 8;
-        hipError_t e = launch_dual((uint32_t)auto_cfg(nb, di.num_cus).wg_waves,
-                                   static_cast<const uint8_t*>(d_input) + (size_t)y0 * 4 * pitch, pitch,
-                                   static_cast<uint8_t*>(d_dxt1) + out_off, static_cast<uint8_t*>(d_etc1) + out_off,
-                                   bw, nb, (hipStream_t)stream, y0 == 0 ? (hipEvent_t)start_event : nullptr,
-                                   y0 + rows == bh ? (hipEvent_t)stop_event : nullptr);
+        const size_t out_off = (size_t)y0 * bw * bb;
+        const uint32_t wg = (uint32_t)auto_cfg(nb, di.num_cus).wg_waves;
+        const uint8_t* in = static_cast<const uint8_t*>(d_input) + (size_t)y0 * 4 * pitch;
+        uint8_t* a = static_cast<uint8_t*>(d_a) + out_off;
+        uint8_t* b = static_cast<uint8_t*>(d_b) + out_off;
+        hipEvent_t e0 = y0 == 0 ? (hipEvent_t)start_event : nullptr;
+        hipEvent_t e1 = y0 + rows == bh ? (hipEvent_t)stop_event : nullptr;
+        hipError_t e = alpha ? launch_dual_alpha(wg, in, pitch, a, b, bw, nb, (hipStream_t)stream, e0, e1)
+                             : launch_dual(wg, in, pitch, a, b, bw, nb, (hipStream_t)stream, e0, e1);
         if (e != hipSuccess) return fail_hip(e, "dual encode kernel launch");
         y0 += rows;
     }
     return GOOFY_OK;
+}
+
+int goofy_hip_compress_device_dual(void* d_dxt1, void* d_etc1, const void* d_input, unsigned int width,
+                                   unsigned int height, size_t pitch, void* stream, void* start_event,
+                                   void* stop_event)
+{
+    return compress_device_pair(false, d_dxt1, d_etc1, d_input, width, height, pitch, stream, start_event,
+                                stop_event);
+}
+
+int goofy_hip_compress_device_dual_alpha(void* d_dxt5, void* d_etc2, const void* d_input, unsigned int width,
+                                         unsigned int height, size_t pitch, void* stream, void* start_event,
+                                         void* stop_event)
+{
+    return compress_device_pair(true, d_dxt5, d_etc2, d_input, width, height, pitch, stream, start_event,
+                                stop_event);
+}
+
+int goofy_hip_compress_dual_alpha(unsigned char* result_dxt5, unsigned char* result_etc2, const unsigned char* input,
+                                  unsigned int width, unsigned int height, unsigned int stride)
+{
+    return compress_host(kHostDualAlpha, result_dxt5, input, width, height, stride, result_etc2);
 }
 
 int goofy_hip_tune_device(int format, void* d_result, const void* d_input, unsigned int width, unsigned int height,

@@ -97,6 +97,9 @@ hipError_t launch_plain(int fmt, const LaunchCfg& cfg, const void* src, size_t p
                         uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_dual(uint32_t wg_waves, const void* src, size_t pitch, void* dst_dxt1, void* dst_etc1, uint32_t bw,
                        uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+hipError_t launch_dual_alpha(uint32_t wg_waves, const void* src, size_t pitch, void* dst_dxt5, void* dst_etc2,
+                             uint32_t bw, uint32_t nblocks, hipStream_t stream, hipEvent_t start = nullptr,
+                             hipEvent_t stop = nullptr);
 // alpha: the work-list holds DXT5 / ETC2 RGBA8 textures too (the kernel
 // instance with all four formats; without, DXT1 / ETC1S only).
 hipError_t launch_batch(const LaunchCfg& cfg, const TexDesc* d_descs, uint32_t ntex, uint32_t ntiles, int num_cus,

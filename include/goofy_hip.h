@@ -93,6 +93,12 @@ GOOFY_HIP_API int goofy_hip_compress_etc2(unsigned char* result, const unsigned 
 GOOFY_HIP_API int goofy_hip_compress_dual(unsigned char* result_dxt1, unsigned char* result_etc1,
                                           const unsigned char* input, unsigned int width, unsigned int height,
                                           unsigned int stride);
+/* The same for the 16-byte formats: DXT5 into result_dxt5 and ETC2 RGBA8 into
+ * result_etc2 (each (width/4)*(height/4)*16 bytes) from one transfer and one
+ * pass, bit-identical to goofy_hip_compress_dxt5 and _etc2. */
+GOOFY_HIP_API int goofy_hip_compress_dual_alpha(unsigned char* result_dxt5, unsigned char* result_etc2,
+                                                const unsigned char* input, unsigned int width, unsigned int height,
+                                                unsigned int stride);
 
 /* ---- device memory, asynchronous ------------------------------------------
  * Encodes a device-resident RGBA8 image into device-resident blocks on
@@ -143,6 +149,13 @@ GOOFY_HIP_API int goofy_hip_compress_device_timed(int format, void* d_result, co
 GOOFY_HIP_API int goofy_hip_compress_device_dual(void* d_dxt1, void* d_etc1, const void* d_input, unsigned int width,
                                                  unsigned int height, size_t pitch, void* stream, void* start_event,
                                                  void* stop_event);
+/* The same for the 16-byte formats: DXT5 into d_dxt5 and ETC2 RGBA8 into
+ * d_etc2 (each (width/4)*(height/4)*16 bytes, 16-byte aligned); the colour
+ * classification and the alpha rows of a block are computed once.  96 bytes
+ * of traffic per block instead of 2 x 80. */
+GOOFY_HIP_API int goofy_hip_compress_device_dual_alpha(void* d_dxt5, void* d_etc2, const void* d_input,
+                                                       unsigned int width, unsigned int height, size_t pitch,
+                                                       void* stream, void* start_event, void* stop_event);
 
 /* Launch-shape autotuner for one image shape.  Times the encode kernel under
  * each candidate shape (1, 2 or 4 blocks per lane; workgroups of 1, 2, 4 or 8

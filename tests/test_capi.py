@@ -86,6 +86,19 @@ def test_format_block_bytes():
     assert [goofy_amd.block_bytes(k) for k in (0, 1, 2, 3)] == [8, 8, 16, 16]
 
 
+def test_dual_alpha_return_codes_without_device():
+    out_a, out_b = np.zeros(64, np.uint8), np.zeros(64, np.uint8)
+    img = np.zeros(64 * 64 * 4, np.uint8)
+    f = goofy_amd._lib.goofy_hip_compress_dual_alpha
+    u8 = goofy_amd._u8
+    assert f(u8(out_a), u8(out_b), u8(img), 60, 64, 240) == -1
+    assert f(u8(out_a), u8(out_b), u8(img), 64, 62, 256) == -2
+    assert f(u8(out_a), u8(None), u8(img), 64, 64, 256) == goofy_amd.ERR_INVALID
+    with pytest.raises(goofy_amd.GoofyError) as e:  # 16-byte blocks need 16-byte aligned outputs
+        goofy_amd.compress_device_dual_alpha(16, 24, 16, 8, 4, 32)
+    assert e.value.status == goofy_amd.ERR_INVALID
+
+
 def test_dual_dropin_return_codes_without_device():
     out_a, out_b = np.zeros(64, np.uint8), np.zeros(64, np.uint8)
     img = np.zeros(64 * 64 * 4, np.uint8)

@@ -193,3 +193,44 @@ def test_alpha_large_fuzz_vs_oracle(gpu, oracle):
         want = oracle.compress_blocks(fmt, img, 4 * bw, 4 * bh, 16 * bw)
         bad = np.nonzero((got.reshape(-1, 16) != want.reshape(-1, 16)).any(1))[0]
         assert bad.size == 0, f"{bad.size} blocks differ, first {bad[:8]}"
+
+
+# ---- DXT5 and ETC2 RGBA8 from one read of the image -----------------------------------------
+
+def test_dual_alpha_golden_digests(gpu, digests):
+    im = W.ALPHA_8192
+    d = digests["alpha_images"][im.name]
+    src = dev_buffer(im.pitch * im.height, fill=0)
+    gpu.synth_device(im.kind, im.seed, im.texture_id, im.noise, src, im.width, im.height, im.pitch)
+    a, b = dev_buffer(im.blocks * 16, fill=0xCD), dev_buffer(im.blocks * 16, fill=0xCD)
+    say(f"dual alpha {im.name}")
+    gpu.compress_device_dual_alpha(a, b, src, im.width, im.height, im.pitch)
+    import hashlib
+    assert hashlib.sha256(to_host(a).tobytes()).hexdigest() == d["dxt5_sha256"]
+    assert hashlib.sha256(to_host(b).tobytes()).hexdigest() == d["etc2_sha256"]
+
+
+@pytest.mark.parametrize("kind,bw,bh", [("alpha", 1024, 256), ("threshold", 300, 77), ("alpha", 1 << 19, 4)])
+def test_dual_alpha_vs_oracle(gpu, oracle, kind, bw, bh):
+    """Random blocks; the last case has 2^19-block rows, run as stripes."""
+    from blockgen import block_image
+    img = block_image(kind, 91 + bh, bw, bh)
+    src = upload(img)
+    n = bw * bh * 16
+    a, b = dev_buffer(n, fill=0xCD), dev_buffer(n, fill=0xCD)
+    say(f"dual alpha {kind} {4 * bw}x{4 * bh}")
+    gpu.compress_device_dual_alpha(a, b, src, 4 * bw, 4 * bh, 16 * bw)
+    assert np.array_equal(to_host(a), oracle.compress_blocks(W.DXT5, img, 4 * bw, 4 * bh, 16 * bw))
+    assert np.array_equal(to_host(b), oracle.compress_blocks(W.ETC2, img, 4 * bw, 4 * bh, 16 * bw))
+
+
+@pytest.mark.parametrize("w,h,extra", [(48, 20, 16), (2048, 1028, 16), (4096, 2052, 0)])
+def test_dual_alpha_host_matches_single_format(gpu, w, h, extra):
+    stride = w * 4 + extra
+    img = gpu.synth_host(W.WAVES_ALPHA, 31, 2, 9, w, h, stride, fill=0x42)
+    say(f"dual alpha host {w}x{h} stride={stride}")
+    a, b = gpu.compress_dual_alpha(img, w, h, stride)
+    n = (w // 4) * (h // 4) * 16
+    one, two = np.full(n, 0xCD, np.uint8), np.full(n, 0xCD, np.uint8)
+    assert gpu.compressDXT5(one, img, w, h, stride) == 0 and gpu.compressETC2(two, img, w, h, stride) == 0
+    assert np.array_equal(a, one) and np.array_equal(b, two)
