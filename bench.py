@@ -513,8 +513,8 @@ def profile_mode(g, hip, W, launches: int):
     """Short run for rocprofv3, all on one stream: each plain kernel `launches`
     times (default launch shape), the traffic probe `launches` times per
     workgroup size (1, 2, 4, 8 waves), the dual-format kernel `launches` times,
-    the DXT5 / ETC2 RGBA8 kernels `launches` times each on the image with an
-    alpha channel, and a device-to-device copy of the 256 MiB image (the
+    the DXT5 / ETC2 RGBA8 kernels and their one-read pair `launches` times each
+    on the image with an alpha channel, and a device-to-device copy of the 256 MiB image (the
     runtime's copy kernel) used to calibrate the HBM byte counters.  Inputs
     rotate over four 256 MiB buffers."""
     im = W.IMAGE_8192
@@ -544,6 +544,9 @@ def profile_mode(g, hip, W, launches: int):
     for f in (W.DXT5, W.ETC2):
         for i in range(launches):
             g.compress_device(f, out16, srcs[i % 4], ia.width, ia.height, ia.pitch)
+    out16b = hip.DeviceBuffer(ia.blocks * 16)
+    for i in range(launches):
+        g.compress_device_dual_alpha(out16, out16b, srcs[i % 4], ia.width, ia.height, ia.pitch)
     hip.synchronize()
     log(f"profile mode done: {launches} launches per kernel")
 
