@@ -17,7 +17,8 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libgoofy_hip.so"
 ARCH = os.environ.get("GOOFY_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "capi.cpp", "batch.cpp", "serve.cpp", "codec_extra.hip", "probe.hip", "containers.cpp"]
+SOURCES = ["kernels.hip", "capi.cpp", "codec_api.cpp", "batch.cpp", "serve.cpp", "codec_extra.hip", "probe.hip",
+           "containers.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
