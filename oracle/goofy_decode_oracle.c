@@ -1,8 +1,9 @@
 /*
  * goofy_decode_oracle.c -- TEST INFRASTRUCTURE ONLY.
  *
- * Independent BC1 and ETC1 block decoders, written from the public format
- * definitions (BC1: two RGB565 endpoints, 2-bit indices LSB-first per pixel
+ * Independent BC1 and ETC1 block decoders (and BC3 / EAC alpha, decoded as
+ * the harness's decoder does, Src/decoder.cpp:679-713, :874-925), written
+ * from the public format definitions (BC1: two RGB565 endpoints, 2-bit indices LSB-first per pixel
  * in row-major order; ETC1: 64-bit big-endian word, individual/differential
  * base colours, eight intensity tables, MSB/LSB selector planes in
  * column-major pixel order).  The rounding of the BC1 interpolants follows

@@ -1,7 +1,8 @@
 /*
  * goofy_oracle.h -- TEST INFRASTRUCTURE ONLY: the CPU oracle of the Goofy
- * DXT1 / ETC1S block encoder (restatement of GoofyTC/goofy_tc.h) plus an
- * independent BC1 / ETC1 block decoder used for round-trip invariants.
+ * DXT1 / ETC1S block encoder (restatement of GoofyTC/goofy_tc.h), the alpha
+ * blocks of the DXT5 / ETC2 RGBA8 forms, and independent BC1 / ETC1 / BC3 /
+ * EAC block decoders used for round-trip invariants.
  * Only tests, the smoke check and the benchmark's CPU-baseline leg use it.
  */
 #ifndef GOOFY_ORACLE_H

@@ -1,7 +1,9 @@
 """TEST INFRASTRUCTURE ONLY: ctypes access to the CPU oracle.
 
-* ``Oracle``  -- the C restatement of GoofyTC/goofy_tc.h (oracle/goofy_oracle.c)
-  plus the independent BC1/ETC1 decoders.
+* ``Oracle``  -- the C restatement of GoofyTC/goofy_tc.h (oracle/goofy_oracle.c),
+  the rule of the DXT5 / ETC2 RGBA8 alpha blocks (no reference encoder; pinned
+  through the harness's decoders by tests/test_alpha.py), and the independent
+  BC1 / ETC1 / BC3 / EAC decoders.
 * ``Reference`` -- the reference header compiled from its own source into
   oracle/_ref/libgoofy_ref.so (present when the reference tree was available
   at build time; the built file travels with the repository snapshot).
