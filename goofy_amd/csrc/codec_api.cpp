@@ -68,33 +68,48 @@ static double psnr_of(double mse, double peak)
 // shared by two calls in flight; no call frees one, because hipFree would wait
 // for every stream on the device (a quality check during serving would stall
 // the server's streams).
+// A quality call's buffers: device scratch (quality_scratch_bytes) and one
+// QualitySums of pinned host memory that the final-sum kernel writes.
+struct QualitySlot {
+    int device;
+    void* d;
+    void* h;
+};
+
 class QualityScratchPool {
 public:
     static constexpr size_t kBytes = kQualityPartialsOffset + (size_t)kQualityMaxGroups * sizeof(QualitySums);
 
-    void* acquire(int device)
+    // A free slot of `device`, or a new one (device scratch and pinned host
+    // memory allocated).  d == nullptr on failure.
+    QualitySlot acquire(int device)
     {
         {
             std::lock_guard<std::mutex> lock(mu_);
             for (size_t i = 0; i < free_.size(); ++i)
-                if (free_[i].first == device) {
-                    void* p = free_[i].second;
+                if (free_[i].device == device) {
+                    const QualitySlot s = free_[i];
                     free_.erase(free_.begin() + (long)i);
-                    return p;
+                    return s;
                 }
         }
-        void* p = nullptr;
-        return hipMalloc(&p, kBytes) == hipSuccess ? p : nullptr;
+        QualitySlot s{device, nullptr, nullptr};
+        if (hipMalloc(&s.d, kBytes) != hipSuccess) return {device, nullptr, nullptr};
+        if (hipHostMalloc(&s.h, sizeof(QualitySums), hipHostMallocDefault) != hipSuccess) {
+            (void)hipFree(s.d);
+            return {device, nullptr, nullptr};
+        }
+        return s;
     }
-    void release(int device, void* p)
+    void release(const QualitySlot& s)
     {
         std::lock_guard<std::mutex> lock(mu_);
-        free_.emplace_back(device, p);
+        free_.push_back(s);
     }
 
 private:
     std::mutex mu_;
-    std::vector<std::pair<int, void*>> free_;
+    std::vector<QualitySlot> free_;
 };
 
 static QualityScratchPool& quality_scratch_pool()
@@ -103,9 +118,9 @@ static QualityScratchPool& quality_scratch_pool()
     return *pool;
 }
 
-// Runs `launch(scratch)` -- a quality launcher (QualitySums at offset 0 of
-// its scratch buffer when done) -- reads the sums back on `stream` and fills
-// `out` with getMsePsnr's figures (Src/main.cpp:403-469).
+// Runs `launch(scratch, host)` -- quality launches on `stream` whose last one
+// leaves the sums at offset 0 of scratch and at host (pinned) -- waits for the
+// stream and fills `out` with getMsePsnr's figures (Src/main.cpp:403-469).
 extern "C++" template <typename Launch>
 static int run_quality(uint64_t pixels, uint64_t units, hipStream_t stream, goofy_quality* out, Launch&& launch)
 {
@@ -113,24 +128,23 @@ static int run_quality(uint64_t pixels, uint64_t units, hipStream_t stream, goof
     DeviceInfo di;
     int rc = device_info(&di);
     if (rc != GOOFY_OK) return rc;
-    void* d = quality_scratch_pool().acquire(di.device);
-    if (!d) return fail(GOOFY_ERR_OUT_OF_MEMORY, "quality scratch");
-    struct Release {  // on every path, the stream has drained before the buffer is reused
-        int dev;
-        void* p;
+    const QualitySlot slot = quality_scratch_pool().acquire(di.device);
+    if (!slot.d) return fail(GOOFY_ERR_OUT_OF_MEMORY, "quality scratch");
+    struct Release {  // on every path, the stream has drained before the slot is reused
+        QualitySlot slot;
         hipStream_t s;
         ~Release()
         {
             (void)hipStreamSynchronize(s);
-            quality_scratch_pool().release(dev, p);
+            quality_scratch_pool().release(slot);
         }
-    } guard{di.device, d, stream};
-    hipError_t e = launch(d);
+    } guard{slot, stream};
+    hipError_t e = launch(slot.d, slot.h);
     if (e != hipSuccess) return fail_hip(e, "quality kernel launch");
-    QualitySums h;
-    e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return fail_hip(e, "quality readback");
+    QualitySums h;
+    std::memcpy(&h, slot.h, sizeof h);
     goofy_quality q{};
     q.pixels = pixels;
     for (int i = 0; i < 4; ++i) q.sse_byte[i] = h.e[i];
@@ -163,15 +177,15 @@ int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_sou
     int rc = check_image_device(format, d_source, d_blocks, width, height, pitch, true);
     if (rc != GOOFY_OK) return rc;
     if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
-    const uint32_t bw = width / 4, bh = height / 4, max_rows = max_stripe_brows(bw);
+    const uint32_t bw = width / 4, bh = height / 4, max_rows = quality_stripe_rows(bw);
     const uint64_t stripe_blocks = (uint64_t)bw * std::min(max_rows, bh);
-    return run_quality((uint64_t)width * height, stripe_blocks, (hipStream_t)stream, out, [&](void* scratch) {
+    return run_quality((uint64_t)width * height, stripe_blocks, (hipStream_t)stream, out, [&](void* scratch, void* host) {
         hipError_t e = hipSuccess;
-        for (uint32_t y0 = 0; y0 < bh && e == hipSuccess;) {  // stripes beyond one launch's index range
+        for (uint32_t y0 = 0; y0 < bh && e == hipSuccess;) {  // stripes beyond one launch's range
             const uint32_t rows = std::min(max_rows, bh - y0);
             e = launch_sqerr(format, static_cast<const uint8_t*>(d_blocks) + (size_t)y0 * bw * block_bytes(format),
                              static_cast<const uint8_t*>(d_source) + (size_t)y0 * 4 * pitch, pitch, width, rows * 4,
-                             scratch, y0 > 0, (hipStream_t)stream);
+                             scratch, y0 > 0, host, (hipStream_t)stream);
             y0 += rows;
         }
         return e;
@@ -206,15 +220,15 @@ int goofy_hip_quality_images_device(const void* d_a, size_t pitch_a, const void*
     int rc = check_rgba_pair(d_a, pitch_a, d_b, pitch_b, width, height, 1);
     if (rc != GOOFY_OK) return rc;
     if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
-    const uint32_t qw = width / 4, max_rows = max_stripe_brows(qw);  // rows of qw 4-pixel units
+    const uint32_t qw = width / 4, max_rows = quality_stripe_rows(qw);  // rows of qw 4-pixel units
     const uint64_t stripe_units = (uint64_t)qw * std::min(max_rows, height);
-    return run_quality((uint64_t)width * height, stripe_units, (hipStream_t)stream, out, [&](void* scratch) {
+    return run_quality((uint64_t)width * height, stripe_units, (hipStream_t)stream, out, [&](void* scratch, void* host) {
         hipError_t e = hipSuccess;
         for (uint32_t y0 = 0; y0 < height && e == hipSuccess;) {
             const uint32_t rows = std::min(max_rows, height - y0);
             e = launch_sqerr_images(static_cast<const uint8_t*>(d_a) + (size_t)y0 * pitch_a, pitch_a,
                                     static_cast<const uint8_t*>(d_b) + (size_t)y0 * pitch_b, pitch_b, width, rows,
-                                    scratch, y0 > 0, (hipStream_t)stream);
+                                    scratch, y0 > 0, host, (hipStream_t)stream);
             y0 += rows;
         }
         return e;

@@ -65,28 +65,61 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ bloc
     }
 }
 
-// Squared error of one pixel pair into e[0..3] (bytes 0..3) and ey (luma).
-// The harness's luma weights are 0.21 / 0.72 / 0.07 on bytes 2 / 1 / 0, so
-// 100 x the luma difference is the integer 21 dr + 72 dg + 7 db (|.| <= 25500);
-// ey sums its square exactly and the host divides by 10^4 once.  That is the
-// exact value of the harness's sum, which its own double arithmetic rounds
-// per pixel (relative differences of order 1e-15 per term).
-__device__ __forceinline__ void accum_pixel(uint32_t a, uint32_t d, uint32_t (&e)[4], uint64_t& ey)
+// Squared errors of pixel pairs, summed exactly in integers.  e[0..3] sum
+// the per-byte squares.  The harness's luma weights are 0.21 / 0.72 / 0.07 on
+// bytes 2 / 1 / 0, so 100 x the luma difference is the integer
+// 21 dr + 72 dg + 7 db, whose square is
+//   441 dr^2 + 5184 dg^2 + 49 db^2 + 3024 dr dg + 294 dr db + 1008 dg db.
+// Beyond the channel squares the luma sum needs only the three cross products
+// x[0] = sum dr dg, x[1] = sum dr db, x[2] = sum dg db (three 24-bit
+// multiply-adds a pixel, against a weighted sum and a 32-bit square);
+// lane_totals() forms the weighted total once per lane.  The host divides it by
+// 10^4 once: the exact value of the harness's sum, which its own double
+// arithmetic rounds per pixel (relative differences of order 1e-15 per term).
+// A lane's sums over at most kQualityLaneUnits blocks (or 4-pixel quads):
+// |x| <= 2048 * 16 * 255^2 < 2^31 and e <= 2048 * 16 * 255^2 < 2^32, so the
+// loop adds in 32 bits and widens once, at the end.
+struct ErrSums {
+    uint32_t e[4];
+    int32_t x[3];
+};
+// c + a * b on the low 24 bits of a and b, as one full-rate v_mad_i32_i24.
+// The compiler would split each into a 24-bit multiply and a shared
+// three-input add (1.5 instructions); the operands here are byte differences.
+__device__ __forceinline__ int32_t mad24(int32_t a, int32_t b, int32_t c)
+{
+    int32_t r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ void accum_pixel(uint32_t a, uint32_t d, ErrSums& s)
 {
     int32_t diff[4];
 #pragma unroll
     for (int ch = 0; ch < 4; ++ch) {
         diff[ch] = (int32_t)((a >> (8 * ch)) & 0xFFu) - (int32_t)((d >> (8 * ch)) & 0xFFu);
-        e[ch] += (uint32_t)(diff[ch] * diff[ch]);
+        s.e[ch] = (uint32_t)mad24(diff[ch], diff[ch], (int32_t)s.e[ch]);
     }
-    const int32_t dy100 = 21 * diff[2] + 72 * diff[1] + 7 * diff[0];
-    ey += (uint64_t)(uint32_t)(dy100 * dy100);  // <= 650,250,000
+    s.x[0] = mad24(diff[2], diff[1], s.x[0]);
+    s.x[1] = mad24(diff[2], diff[0], s.x[1]);
+    s.x[2] = mad24(diff[1], diff[0], s.x[2]);
+}
+// The lane's channel sums and its sum of squared luma differences (x 10^4),
+// in 64 bits; the luma total is exact and non-negative.
+__device__ __forceinline__ void lane_totals(const ErrSums& s, unsigned long long (&e)[4], unsigned long long& ey)
+{
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) e[ch] = s.e[ch];
+    ey = (unsigned long long)(441ll * s.e[2] + 5184ll * s.e[1] + 49ll * s.e[0] + 3024ll * s.x[0] + 294ll * s.x[1] +
+                              1008ll * s.x[2]);
 }
 
 // Sums of one workgroup (256 lanes = 4 waves): wave reduction by lane
-// exchange, then the four wave sums, stored by lane 0 with a plain vector
-// store.  Every lane of the workgroup must reach this.
-__device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], unsigned long long ey, QualitySums* out)
+// exchange, then the four wave sums, stored by thread 0 at out and, if given,
+// at host_out.  Every lane of the workgroup must reach this.
+__device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], unsigned long long ey,
+                                                 QualitySums* __restrict__ out,
+                                                 QualitySums* __restrict__ host_out = nullptr)
 {
     __shared__ unsigned long long s_e[4][4];
     __shared__ unsigned long long s_y[4];
@@ -109,6 +142,7 @@ __device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], uns
         for (int ch = 0; ch < 4; ++ch) r.e[ch] = s_e[0][ch] + s_e[1][ch] + s_e[2][ch] + s_e[3][ch];
         r.y100 = s_y[0] + s_y[1] + s_y[2] + s_y[3];
         *out = r;
+        if (host_out) *host_out = r;
     }
 }
 
@@ -123,12 +157,11 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint8_t* __restrict__ block
                                                uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
                                                QualitySums* __restrict__ part)
 {
-    unsigned long long acc[4] = {0, 0, 0, 0};
-    uint64_t ey = 0;
+    ErrSums acc = {};
     const uint64_t step = (uint64_t)gridDim.x * 256u;
     // Software pipeline: the loads of a lane's next block are issued before
     // the current block is decoded and compared, so the reads stay in flight
-    // across the arithmetic.
+    // across the arithmetic.  nblocks >= 1 (the launcher's check).
     auto fetch = [&](uint64_t i, uint4& b, u32x4 (&v)[4]) {
         const uint32_t by = (uint32_t)((i * magic) >> 40), bx = (uint32_t)i - by * bw;
         const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
@@ -137,29 +170,38 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint8_t* __restrict__ block
         for (int r = 0; r < 4; ++r)
             v[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
     };
-    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    uint4 nb = make_uint4(0u, 0u, 0u, 0u);
-    u32x4 nv[4] = {};
-    if (i < nblocks) fetch(i, nb, nv);
-    while (i < nblocks) {
-        const uint4 b = nb;
-        const u32x4 v[4] = {nv[0], nv[1], nv[2], nv[3]};
-        const uint64_t next = i + step;
-        if (next < nblocks) fetch(next, nb, nv);
+    auto compare = [&](const uint4& b, const u32x4 (&v)[4]) {
         uint32_t px[16];
         decode_block<FMT>(b, px);
-        uint32_t e[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const uint32_t s[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
-#pragma unroll
-            for (int c = 0; c < 4; ++c) accum_pixel(s[c], px[4 * r + c], e, ey);
+            accum_pixel(v[r].x, px[4 * r], acc);
+            accum_pixel(v[r].y, px[4 * r + 1], acc);
+            accum_pixel(v[r].z, px[4 * r + 2], acc);
+            accum_pixel(v[r].w, px[4 * r + 3], acc);
         }
-#pragma unroll
-        for (int ch = 0; ch < 4; ++ch) acc[ch] += e[ch];
-        i = next;
+    };
+    // Two register sets used in turn, so that no block is copied from one
+    // set to the other; a prefetch past the end reloads the last block
+    // (in bounds, never compared).
+    const uint64_t last = nblocks - 1u;
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;  // i >= nblocks: the loop does not run
+    uint4 b0, b1;
+    u32x4 v0[4], v1[4];
+    fetch(i < last ? i : last, b0, v0);
+    while (i < nblocks) {
+        const uint64_t i1 = i + step;
+        fetch(i1 < last ? i1 : last, b1, v1);
+        compare(b0, v0);
+        if (i1 >= nblocks) break;
+        const uint64_t i2 = i1 + step;
+        fetch(i2 < last ? i2 : last, b0, v0);
+        compare(b1, v1);
+        i = i2;
     }
-    block_sums_store(acc, ey, part + blockIdx.x);
+    unsigned long long e[4], ey;
+    lane_totals(acc, e, ey);
+    block_sums_store(e, ey, part + blockIdx.x);
 }
 
 // Squared errors of two whole RGBA8 images (getMsePsnr over a source and any
@@ -169,44 +211,54 @@ __global__ __launch_bounds__(256) void k_sqerr_images(const uint8_t* __restrict_
                                                       uint32_t nquads, uint64_t magic,
                                                       QualitySums* __restrict__ part)
 {
-    unsigned long long acc[4] = {0, 0, 0, 0};
-    uint64_t ey = 0;
+    ErrSums acc = {};
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < nquads; i += (uint64_t)gridDim.x * 256u) {
         const uint32_t y = (uint32_t)((i * magic) >> 40), x = (uint32_t)i - y * qw;
         const u32x4 va =
             __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a + (uint64_t)y * pitch_a + (uint64_t)x * 16u));
         const u32x4 vb =
             __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(b + (uint64_t)y * pitch_b + (uint64_t)x * 16u));
-        uint32_t e[4] = {0, 0, 0, 0};
-        accum_pixel(va.x, vb.x, e, ey);
-        accum_pixel(va.y, vb.y, e, ey);
-        accum_pixel(va.z, vb.z, e, ey);
-        accum_pixel(va.w, vb.w, e, ey);
-#pragma unroll
-        for (int ch = 0; ch < 4; ++ch) acc[ch] += e[ch];
+        accum_pixel(va.x, vb.x, acc);
+        accum_pixel(va.y, vb.y, acc);
+        accum_pixel(va.z, vb.z, acc);
+        accum_pixel(va.w, vb.w, acc);
     }
-    block_sums_store(acc, ey, part + blockIdx.x);
+    unsigned long long e[4], ey;
+    lane_totals(acc, e, ey);
+    block_sums_store(e, ey, part + blockIdx.x);
 }
 
-// One workgroup adds the g partials in a fixed order into *out; with
-// accumulate, onto the sums already in *out (the stripes of a large image,
-// launched one after another on one stream).
+// One workgroup adds the g <= kQualityMaxGroups partials in a fixed order into
+// *out -- with accumulate, onto the sums already in *out (the stripes of a
+// large image, launched one after another on one stream) -- and copies the
+// total to host_out (pinned host memory, read once the stream has drained)
+// when given.  A lane issues the loads of all its partials before it adds any.
 __global__ __launch_bounds__(256) void k_sums_final(const QualitySums* __restrict__ part, uint32_t g,
-                                                    QualitySums* __restrict__ out, uint32_t accumulate)
+                                                    QualitySums* __restrict__ out, uint32_t accumulate,
+                                                    QualitySums* __restrict__ host_out)
 {
+    constexpr uint32_t kPer = kQualityMaxGroups / 256u;
+    static_assert(kQualityMaxGroups % 256u == 0, "partials per lane");
+    QualitySums p[kPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t k = threadIdx.x + 256u * j;
+        p[j] = k < g ? part[k] : QualitySums{{0, 0, 0, 0}, 0};
+    }
     unsigned long long acc[4] = {0, 0, 0, 0};
-    uint64_t ey = 0;
-    if (accumulate && threadIdx.x == 0) {  // read before lane 0 writes the new sums at the end
+    unsigned long long ey = 0;
+    if (accumulate && threadIdx.x == 0) {  // read before thread 0 writes the new sums at the end
 #pragma unroll
         for (int ch = 0; ch < 4; ++ch) acc[ch] = out->e[ch];
         ey = out->y100;
     }
-    for (uint32_t k = threadIdx.x; k < g; k += 256u) {
 #pragma unroll
-        for (int ch = 0; ch < 4; ++ch) acc[ch] += part[k].e[ch];
-        ey += part[k].y100;
+    for (uint32_t j = 0; j < kPer; ++j) {
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) acc[ch] += p[j].e[ch];
+        ey += p[j].y100;
     }
-    block_sums_store(acc, ey, out);
+    block_sums_store(acc, ey, out, host_out);
 }
 
 // One 2x2 cell of the quality floor (Src/main.cpp:507-553).  p and q are the
@@ -270,9 +322,10 @@ hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width,
 }
 
 hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
-                        void* scratch, bool accumulate, hipStream_t stream)
+                        void* scratch, bool accumulate, void* host_out, hipStream_t stream)
 {
     const uint32_t bw = width / 4, nblocks = bw * (height / 4);
+    if (nblocks == 0 || nblocks > kQualityMaxLaunchUnits) return hipErrorInvalidValue;  // 32-bit lane sums
     const uint32_t g = quality_grid(nblocks);
     QualitySums* out = static_cast<QualitySums*>(scratch);
     QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(scratch) + kQualityPartialsOffset);
@@ -286,21 +339,22 @@ hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pit
     default: hipLaunchKernelGGL(k_sqerr<3>, dim3(g), dim3(256), 0, stream, b, s, (uint64_t)pitch, bw, nblocks, magic, part);
     }
     hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out,
-                       accumulate ? 1u : 0u);
+                       accumulate ? 1u : 0u, static_cast<QualitySums*>(host_out));
     return hipGetLastError();
 }
 
 hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
-                               uint32_t height, void* scratch, bool accumulate, hipStream_t stream)
+                               uint32_t height, void* scratch, bool accumulate, void* host_out, hipStream_t stream)
 {
     const uint32_t qw = width / 4, n = qw * height;
+    if (n == 0 || n > kQualityMaxLaunchUnits) return hipErrorInvalidValue;  // 32-bit lane sums
     const uint32_t g = quality_grid(n);
     QualitySums* out = static_cast<QualitySums*>(scratch);
     QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(scratch) + kQualityPartialsOffset);
     hipLaunchKernelGGL(k_sqerr_images, dim3(g), dim3(256), 0, stream, (const uint8_t*)a, (uint64_t)pitch_a,
                        (const uint8_t*)b, (uint64_t)pitch_b, qw, n, div_magic(qw), part);
     hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out,
-                       accumulate ? 1u : 0u);
+                       accumulate ? 1u : 0u, static_cast<QualitySums*>(host_out));
     return hipGetLastError();
 }
 

@@ -130,12 +130,24 @@ inline size_t quality_scratch_bytes(uint64_t units)
 {
     return kQualityPartialsOffset + (size_t)quality_grid(units) * sizeof(QualitySums);
 }
+// A quality lane adds at most kQualityLaneUnits blocks (or 4-pixel quads) in
+// 32-bit sums, so one launch covers at most this many units; the callers
+// launch larger images as stripes of quality_stripe_rows(units per row) rows.
+constexpr uint64_t kQualityLaneUnits = 2048;
+constexpr uint64_t kQualityMaxLaunchUnits = kQualityLaneUnits * kQualityMaxGroups * 256u;
+inline uint32_t quality_stripe_rows(uint64_t units_per_row)
+{
+    const uint64_t r = max_stripe_brows(units_per_row), cap = kQualityMaxLaunchUnits / (units_per_row ? units_per_row : 1);
+    return (uint32_t)(r < cap ? r : cap);
+}
 // accumulate: add onto the sums already at offset 0 of scratch (later
 // stripes of one image on the same stream) instead of replacing them.
+// host_out: null, or pinned host memory for a copy of the sums at offset 0
+// (one QualitySums), valid once the stream has drained.
 hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
-                        void* scratch, bool accumulate, hipStream_t stream);
+                        void* scratch, bool accumulate, void* host_out, hipStream_t stream);
 hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
-                               uint32_t height, void* scratch, bool accumulate, hipStream_t stream);
+                               uint32_t height, void* scratch, bool accumulate, void* host_out, hipStream_t stream);
 hipError_t launch_rgb565_baseline(const void* src, size_t src_pitch, void* dst, size_t dst_pitch, uint32_t width,
                                   uint32_t height, hipStream_t stream, hipEvent_t start = nullptr,
                                   hipEvent_t stop = nullptr);
