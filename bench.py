@@ -514,9 +514,10 @@ def profile_mode(g, hip, W, launches: int):
     times (default launch shape), the traffic probe `launches` times per
     workgroup size (1, 2, 4, 8 waves), the dual-format kernel `launches` times,
     the DXT5 / ETC2 RGBA8 kernels and their one-read pair `launches` times each
-    on the image with an alpha channel, and a device-to-device copy of the 256 MiB image (the
-    runtime's copy kernel) used to calibrate the HBM byte counters.  Inputs
-    rotate over four 256 MiB buffers."""
+    on the image with an alpha channel, the four decoders and the four round-trip
+    quality calls `launches` times each on that image, and a device-to-device copy
+    of the 256 MiB image (the runtime's copy kernel) used to calibrate the HBM byte
+    counters.  Encoder inputs rotate over four 256 MiB buffers."""
     im = W.IMAGE_8192
     nbytes = im.pitch * im.height
     srcs = [hip.DeviceBuffer(nbytes) for _ in range(4)]
@@ -547,6 +548,20 @@ def profile_mode(g, hip, W, launches: int):
     out16b = hip.DeviceBuffer(ia.blocks * 16)
     for i in range(launches):
         g.compress_device_dual_alpha(out16, out16b, srcs[i % 4], ia.width, ia.height, ia.pitch)
+    hip.synchronize()
+    # the decoders and the round-trip quality sums, all four formats on the image with alpha
+    # (blocks of srcs[0]): `launches` decodes per format, then `launches` quality calls
+    src = srcs[0]
+    sets = ((W.DXT1, out), (W.ETC1, out2), (W.DXT5, out16), (W.ETC2, out16b))
+    for f, blk in sets:
+        g.compress_device(f, blk, src, ia.width, ia.height, ia.pitch)
+    img = copy_dst
+    for f, blk in sets:
+        for _ in range(launches):
+            g.decode_device(f, img, blk, ia.width, ia.height, ia.pitch)
+    for f, blk in sets:
+        for _ in range(launches):
+            g.quality_device(f, blk, src, ia.width, ia.height, ia.pitch)
     hip.synchronize()
     log(f"profile mode done: {launches} launches per kernel")
 

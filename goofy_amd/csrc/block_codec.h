@@ -34,6 +34,11 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 // Per-byte rounding average of two packed words: v_lerp_u8 with a carry-in
 // of one in every byte, i.e. (a+b+1)>>1 per byte.
 __device__ __forceinline__ uint32_t avg4(uint32_t a, uint32_t b) { return __builtin_amdgcn_lerp(a, b, 0x01010101u); }
+// a ^ b ^ c as one v_bitop3_b32 (truth table 0x96, the same for every operand order).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96u);
+}
 
 // v_perm_b32: byte i of the result is byte sel[i] of {hi:lo} (0..3 = lo,
 // 4..7 = hi, 0x0c = zero).
@@ -384,8 +389,8 @@ __device__ __forceinline__ Rank rank_bits(uint32_t w, const uint32_t (&nthr4)[7]
     for (int k = 0; k < 7; ++k) m[k] = avg4(nthr4[k], w);  // avg(~t, v) >= 128 <=> v >= t
     Rank r;
     r.t2 = m[3];
-    r.t1 = m[1] ^ m[3] ^ m[5];
-    r.t0 = m[0] ^ m[2] ^ m[4] ^ m[6] ^ r.t1;
+    r.t1 = xor3(m[1], m[3], m[5]);
+    r.t0 = xor3(xor3(m[0], m[2], m[4]), m[6], r.t1);
     return r;
 }
 

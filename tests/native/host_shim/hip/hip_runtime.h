@@ -53,5 +53,17 @@ inline uint32_t goofy_host_lerp(uint32_t a, uint32_t b, uint32_t c)
     return out;
 }
 
+// V_BITOP3_B32: bit i of the result is bit (a_i << 2 | b_i << 1 | c_i) of imm.
+inline uint32_t goofy_host_bitop3(uint32_t a, uint32_t b, uint32_t c, uint32_t imm)
+{
+    uint32_t out = 0;
+    for (int i = 0; i < 32; ++i) {
+        const uint32_t k = (((a >> i) & 1u) << 2) | (((b >> i) & 1u) << 1) | ((c >> i) & 1u);
+        out |= ((imm >> k) & 1u) << i;
+    }
+    return out;
+}
+
 #define __builtin_amdgcn_perm goofy_host_perm
 #define __builtin_amdgcn_lerp goofy_host_lerp
+#define __builtin_amdgcn_bitop3_b32 goofy_host_bitop3
