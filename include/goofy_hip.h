@@ -185,10 +185,11 @@ GOOFY_HIP_API int goofy_hip_decode_device(int format, void* d_rgba, const void* 
  * a pixel is read as a little-endian word, "r" = bits 16..23, "g" = 8..15,
  * "b" = 0..7, "a" = 24..31; luma = 0.21 r + 0.72 g + 0.07 b; PSNR =
  * 10 log10(peak^2 / mse) with peak 255, and 768 for the summed-RGB figure;
- * an MSE below 1e-7 gives DBL_MAX.  Synchronous on `stream`.  Uses an
- * 60 KiB device scratch buffer from a per-device pool (allocated on first
- * use, never freed, one per concurrent call), so a call never waits on other
- * streams. */
+ * an MSE below 1e-7 gives DBL_MAX.  Synchronous on `stream`.  Uses a
+ * 60 KiB device scratch buffer and 40 bytes of pinned host memory, into which
+ * the last kernel writes the sums, from a per-device pool (allocated on first
+ * use, never freed, one pair per concurrent call), so a call never waits on
+ * other streams. */
 typedef struct goofy_quality {
     uint64_t pixels;
     uint64_t sse_byte[4]; /* squared error of bytes 0..3 of each pixel, summed */
