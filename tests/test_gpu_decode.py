@@ -202,6 +202,35 @@ def test_floor_and_quality_real_images_match_harness(gpu, real_images, name):
         _check_quality(gpu.quality_device(fmt, blocks, d_src, w, h), g[f"{key}_quality"], w * h)
 
 
+def test_quality_sums_past_one_launch_closed_form(gpu):
+    """819,200,000 blocks (4096 x 3,200,000 pixels, a 52.4 GB source): more than
+    the 805,306,368 blocks that one round-trip launch may cover with 32-bit lane
+    sums (2048 blocks per lane), so the call runs as two stripes, the second
+    adding onto the first.  A white source against all-zero DXT1 blocks (c0 =
+    c1 = 0: 3-colour mode, index 0, opaque black) errs by 255 in bytes 0..2 of
+    every pixel.  That is the largest per-block error, so the lanes' sums sit
+    at the top of their range; in one unstriped launch (2084 blocks per lane)
+    the signed cross-product sums would wrap.  Closed form: 65025 x pixels on
+    bytes 0..2, 0 on alpha, and 65025 x pixels of luma error
+    ((21 + 72 + 7) x 255 = 25500 per pixel, squared, / 10^4)."""
+    from goofy_amd import hip
+    w, h = 4096, 3_200_000
+    pixels = w * h
+    print(f"quality past one launch: {w}x{h}, {pixels // 16} blocks", flush=True)
+    src = hip.DeviceBuffer(w * 4 * h, fill=0xFF)
+    blocks = hip.DeviceBuffer((w // 4) * (h // 4) * 8, fill=0)
+    try:
+        q = gpu.quality_device(W.DXT1, blocks, src, w, h, w * 4)
+    finally:
+        src.free()
+        blocks.free()
+    assert q["pixels"] == pixels
+    assert q["sse_byte"] == [65025 * pixels] * 3 + [0]
+    assert q["sse_luma"] == 65025.0 * pixels
+    assert q["psnr_a"] > 1e300 and q["psnr_rgb"] == pytest.approx(10 * math.log10(768.0 ** 2 / (3 * 65025.0)),
+                                                                  abs=1e-9)
+
+
 def test_quality_images_closed_form(gpu):
     from goofy_amd import hip
     w, h = 64, 8
