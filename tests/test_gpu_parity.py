@@ -535,6 +535,37 @@ def test_dropin_many_chunks_matches_device(gpu, w, h, extra):
     assert np.array_equal(b, encode_dev(gpu, W.ETC1, src, w, h, stride))
 
 
+@pytest.mark.parametrize("w,h,extra", [(528, 1000, 16), (1040, 1004, 48), (2032, 1028, 0)])
+def test_dropin_zero_copy_parts_match_device(gpu, w, h, extra):
+    """Host images of one staging chunk (at most about 8 MiB) run zero-copy,
+    staged and launched in up to 8 parts of at least 512 KiB.  These shapes
+    split into 4, 7 and 8 parts whose block rows do not divide evenly (the last
+    part short), one with padded rows: every format, and both pairs from one
+    call, against the device entry point."""
+    stride = w * 4 + extra
+    img = gpu.synth_host(2, 777, 3, 16, w, h, stride, fill=0x5A)  # waves with alpha
+    src = upload(img)
+    nb = (w // 4) * (h // 4)
+    for fmt, fn in ((W.DXT1, gpu.compressDXT1), (W.ETC1, gpu.compressETC1), (W.DXT5, gpu.compressDXT5),
+                    (W.ETC2, gpu.compressETC2)):
+        say(f"drop-in zero-copy parts fmt={fmt} {w}x{h} stride={stride}")
+        bb = gpu.block_bytes(fmt)
+        dev = dev_buffer(nb * bb, fill=0xCD)
+        gpu.compress_device(fmt, dev, src, w, h, stride)
+        want = to_host(dev)[: nb * bb]
+        out = np.full(nb * bb, 0xCD, np.uint8)
+        assert fn(out, img, w, h, stride) == 0
+        assert np.array_equal(out, want)
+    a, b = gpu.compress_dual(img, w, h, stride)
+    assert np.array_equal(a, encode_dev(gpu, W.DXT1, src, w, h, stride))
+    assert np.array_equal(b, encode_dev(gpu, W.ETC1, src, w, h, stride))
+    c, d = gpu.compress_dual_alpha(img, w, h, stride)
+    for fmt, got in ((W.DXT5, c), (W.ETC2, d)):
+        dev = dev_buffer(nb * 16, fill=0xCD)
+        gpu.compress_device(fmt, dev, src, w, h, stride)
+        assert np.array_equal(got, to_host(dev)[: nb * 16])
+
+
 def test_dropin_from_threads(gpu):
     """The drop-in on four host threads at once, each with its own images of
     one to several staging chunks: every call bit-identical to the same call
