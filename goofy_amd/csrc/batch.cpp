@@ -46,13 +46,31 @@ struct goofy_batch {
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // Recorded after every graph launch, on the caller's stream: the work-list
+    // is rewritten or freed only once the launches that read it have finished.
+    hipEvent_t launched = nullptr;
+    bool launched_pending = false;
 };
 
 namespace {
 
+// Waits for the plan's last graph launch (which may be on a caller's
+// non-blocking stream, which a synchronous copy does not wait for).
+int wait_launched(goofy_batch* b)
+{
+    if (!b->launched_pending) return GOOFY_OK;
+    const hipError_t e = hipEventSynchronize(b->launched);
+    if (e != hipSuccess) return fail_hip(e, "hipEventSynchronize(batch launch)");
+    b->launched_pending = false;
+    return GOOFY_OK;
+}
+
 void drop_graph(goofy_batch* b)
 {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->exec) {
+        (void)wait_launched(b);
+        (void)hipGraphExecDestroy(b->exec);
+    }
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
@@ -61,6 +79,8 @@ void drop_graph(goofy_batch* b)
 // Rebuild the tile prefix sums for the plan's tile size and upload them.
 int build_worklist(goofy_batch* b)
 {
+    int rc = wait_launched(b);
+    if (rc != GOOFY_OK) return rc;
     const uint64_t tile = (uint64_t)tile_blocks(b->cfg);
     b->descs.clear();
     uint64_t tiles = 0;
@@ -276,6 +296,10 @@ int goofy_batch_launch(goofy_batch* b, void* stream)
     if (rc != GOOFY_OK) return rc;
     hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
     if (e != hipSuccess) return fail_hip(e, "hipGraphLaunch");
+    if (!b->launched && (e = hipEventCreateWithFlags(&b->launched, hipEventDisableTiming)) != hipSuccess)
+        return fail_hip(e, "hipEventCreate");
+    if ((e = hipEventRecord(b->launched, (hipStream_t)stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
+    b->launched_pending = true;
     return GOOFY_OK;
 }
 
@@ -284,10 +308,20 @@ int goofy_batch_launch_per_texture(goofy_batch* b, void* stream)
     clear_error();
     if (!b) return fail(GOOFY_ERR_INVALID, "null batch");
     for (const goofy_texture_desc& t : b->textures) {
-        const uint32_t bw = t.width / 4, bh = t.height / 4;
-        const LaunchCfg c = auto_cfg((uint64_t)bw * bh, b->num_cus);
-        hipError_t e = launch_plain(t.format, c, t.src, t.pitch, t.dst, bw, bw * bh, (hipStream_t)stream);
-        if (e != hipSuccess) return fail_hip(e, "per-texture launch");
+        // horizontal stripes of at most max_stripe_brows block rows, as in
+        // the work-list: beyond that a launch's block index or its division
+        // magic would be out of range
+        const uint32_t bw = t.width / 4, bh = t.height / 4, max_rows = max_stripe_brows(bw);
+        for (uint32_t y0 = 0; y0 < bh;) {
+            const uint32_t rows = std::min(max_rows, bh - y0);
+            const uint32_t nb = rows * bw;
+            const LaunchCfg c = auto_cfg(nb, b->num_cus);
+            hipError_t e = launch_plain(t.format, c, static_cast<const uint8_t*>(t.src) + (size_t)y0 * 4 * t.pitch,
+                                        t.pitch, static_cast<uint8_t*>(t.dst) + (size_t)y0 * bw * block_bytes(t.format),
+                                        bw, nb, (hipStream_t)stream);
+            if (e != hipSuccess) return fail_hip(e, "per-texture launch");
+            y0 += rows;
+        }
     }
     return GOOFY_OK;
 }
@@ -398,6 +432,8 @@ int goofy_batch_destroy(goofy_batch* b)
 {
     clear_error();
     if (!b) return GOOFY_OK;
+    (void)wait_launched(b);
+    if (b->launched) (void)hipEventDestroy(b->launched);
     drop_graph(b);
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     if (b->d_descs) (void)hipFree(b->d_descs);

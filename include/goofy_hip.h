@@ -257,7 +257,9 @@ GOOFY_HIP_API int goofy_batch_create(const goofy_texture_desc* textures, size_t 
                                      const goofy_launch_config* cfg, goofy_batch** out);
 /* Enqueue the whole batch on `stream` (replays the captured graph). */
 GOOFY_HIP_API int goofy_batch_launch(goofy_batch* batch, void* stream);
-/* Same work as individual launches (one plain kernel per texture), for comparison. */
+/* Same work as individual launches (one plain kernel per texture, or per
+ * horizontal stripe for textures beyond one launch's index range), for
+ * comparison. */
 GOOFY_HIP_API int goofy_batch_launch_per_texture(goofy_batch* batch, void* stream);
 /* Time the candidate launch shapes on this batch (HIP events, `reps` runs
  * each), keep the fastest in the plan and the process-wide tuning cache, and
@@ -265,6 +267,9 @@ GOOFY_HIP_API int goofy_batch_launch_per_texture(goofy_batch* batch, void* strea
 GOOFY_HIP_API int goofy_batch_autotune(goofy_batch* batch, void* stream, int reps, goofy_launch_config* chosen,
                                        float* best_ms);
 GOOFY_HIP_API int goofy_batch_get_config(const goofy_batch* batch, goofy_launch_config* cfg);
+/* set_config, autotune and destroy first wait for the plan's earlier
+ * goofy_batch_launch calls to finish (on whatever stream they were issued),
+ * since they rewrite or free the device work-list those launches read. */
 GOOFY_HIP_API int goofy_batch_set_config(goofy_batch* batch, const goofy_launch_config* cfg);
 /* Totals of the plan: textures, 4x4 blocks, wave tiles, pixels. */
 GOOFY_HIP_API int goofy_batch_info(const goofy_batch* batch, uint64_t* textures, uint64_t* blocks, uint64_t* tiles,

@@ -204,6 +204,47 @@ def test_images_beyond_one_launch_are_striped(gpu, oracle):
     assert np.array_equal(to_host(floor), oracle.rgb565_baseline(img, w, h))
 
 
+def test_non_power_of_two_rows_striped(gpu, oracle):
+    """4,194,300 x 16 pixels: 1,048,575 blocks per row (not a power of two, so
+    the 40-bit division magic is inexact beyond 2^40 / bw blocks), one block
+    row per launch.  Encode, the batch launch, the per-texture launches of a
+    batch, decode, the quality sums and the RGB565 floor, bit-exact."""
+    from goofy_amd import hip
+    w, h = 4 * 1048575, 16
+    pitch = w * 4
+    rng = np.random.default_rng(22)
+    img = rng.integers(0, 256, pitch * h, dtype=np.uint8)
+    src = upload(img)
+    for fmt in (W.DXT1, W.ETC2):
+        bb = 16 if fmt == W.ETC2 else 8
+        say(f"non-power-of-two stripes {w}x{h} fmt={fmt}: encode")
+        want = oracle.compress_blocks(fmt, img, w, h, pitch)
+        got_buf = dev_buffer(want.size, fill=0xCD)
+        gpu.compress_device(fmt, got_buf, src, w, h, pitch)
+        assert np.array_equal(to_host(got_buf), want)
+        say(f"non-power-of-two stripes {w}x{h} fmt={fmt}: batch, per-texture launches")
+        for per_texture in (False, True):
+            got_buf.fill(0xCD)
+            b = gpu.Batch([dict(src=src, dst=got_buf, width=w, height=h, pitch=pitch, format=fmt)])
+            b.launch_per_texture() if per_texture else b.launch()
+            hip.synchronize()
+            assert np.array_equal(to_host(got_buf), want), per_texture
+            b.close()
+        say(f"non-power-of-two stripes {w}x{h} fmt={fmt}: decode, quality")
+        dec = dev_buffer(pitch * h, fill=0x3C)
+        gpu.decode_device(fmt, dec, got_buf, w, h, pitch)
+        dec_want = oracle.decode_image(fmt, want, w, h)
+        assert np.array_equal(to_host(dec), dec_want)
+        d = img.reshape(-1, 4).astype(np.int64) - dec_want.reshape(-1, 4)
+        assert gpu.quality_device(fmt, got_buf, src, w, h, pitch)["sse_byte"] == [int(x) for x in (d * d).sum(0)]
+        assert want.size == (w // 4) * (h // 4) * bb
+        del dec
+    say(f"non-power-of-two stripes {w}x{h}: rgb565 floor")
+    floor = dev_buffer(pitch * h, fill=0)
+    gpu.rgb565_baseline_device(floor, src, w, h)
+    assert np.array_equal(to_host(floor), oracle.rgb565_baseline(img, w, h))
+
+
 def test_65536_wide_image_striped_vs_reference(gpu, reference):
     """65536 x 16384 pixels (4 GiB, generated on the device): blocks x blocks per
     row is just above 2^40, so the encode runs as 2 stripes.  Both formats,

@@ -93,3 +93,17 @@ def test_device_block_decoders_on_host_match_oracle(tmp_path, oracle, harness, c
     assert np.array_equal(got, img.reshape(n, 64))
     ref = harness.decompress(fmt, blocks.reshape(-1), 4 * n, 4).reshape(4, n, 16).transpose(1, 0, 2)
     assert np.array_equal(got, ref.reshape(n, 64))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_stripe_division_magic_on_host(tmp_path):
+    """internal.h's launch splitting: every stripe of max_stripe_brows(bw) rows
+    indexes its blocks exactly, for powers of two, their neighbours and
+    non-power-of-two widths near 2^19 and 2^20 (tests/native/stripe_magic.cpp)."""
+    exe = tmp_path / "stripe_magic"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    f"-I{ROOT / 'include'}", f"-I{ROOT / 'goofy_amd' / 'csrc'}",
+                    str(ROOT / "tests" / "native" / "stripe_magic.cpp"), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "stripe magic ok" in r.stdout
