@@ -9,10 +9,10 @@
 // column-major pixel order, flip bit), BC3 alpha from :874-925 (8 entries at
 // sevenths when a0 > a1, else 6 at fifths plus 0 and 255, truncating) with
 // its colour block always in 4-colour mode (:945-956), and EAC alpha from
-// :679-713 with the tables of :25-42.  ETC1 blocks whose differential sums
-// leave 0..31 are ETC2 T/H/planar blocks, which the encoder never produces;
-// they are decoded here with 5-bit wrap-around and are outside the parity
-// claim.
+// :679-713 with the tables of :25-42.  A differential ETC block whose red,
+// green or blue sum leaves 0..31 is an ETC2 T, H or planar block (checked in
+// that order, Src/decoder.cpp:716-773; modes at :270-386); both ETC formats
+// decode all five modes, as the harness does.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -62,27 +62,107 @@ __device__ __forceinline__ void decode_bc1(uint32_t w0, uint32_t w1, uint32_t (&
     }
 }
 
-// 16 decoded pixels (row-major) of an ETC1 block; w0/w1 are the block's
-// first/second 4 bytes read as little-endian words.
+// Byte clamp of a small signed sum.
+__device__ __forceinline__ uint32_t clamp_u8(int32_t v) { return (uint32_t)min(max(v, 0), 255); }
+
+// ETC2 T mode (tmode) or H mode: the palette word of each channel's four
+// paint colours, entry = 2 * MSB + LSB of a pixel's selector bits.  hi = the
+// block's first four bytes as a big-endian word (block bits 63..32).
+//   T: colour A red = bits 28-27 and 25-24 of hi, green 23-20, blue 19-16;
+//      colour B red 15-12, green 11-8, blue 7-4; distance index = bits 3-2
+//      then bit 0.  Paint colours A, B + d, B, B - d.
+//   H: colour A red = 30-27, green 26-24 then 20, blue 19 then 17-15;
+//      colour B red 14-11, green 10-7, blue 6-3; distance index = bit 2, bit
+//      0, then 1 if A >= B as 12-bit RGB444 numbers.  Paint colours A + d,
+//      A - d, B + d, B - d.
+// Colours are 4 bits per channel widened by replication (x 17); distances
+// 3, 6, 11, 16, 23, 32, 41, 64.
+__device__ __forceinline__ void etc2_th_planes(uint32_t hi, bool tmode, uint32_t (&plane)[3])
+{
+    uint32_t a4[3], b4[3], di;
+    if (tmode) {
+        a4[0] = (((hi >> 27) & 3u) << 2) | ((hi >> 24) & 3u);
+        a4[1] = (hi >> 20) & 15u;
+        a4[2] = (hi >> 16) & 15u;
+        b4[0] = (hi >> 12) & 15u;
+        b4[1] = (hi >> 8) & 15u;
+        b4[2] = (hi >> 4) & 15u;
+        di = (((hi >> 2) & 3u) << 1) | (hi & 1u);
+    } else {
+        a4[0] = (hi >> 27) & 15u;
+        a4[1] = (((hi >> 24) & 7u) << 1) | ((hi >> 20) & 1u);
+        a4[2] = (((hi >> 19) & 1u) << 3) | ((hi >> 15) & 7u);
+        b4[0] = (hi >> 11) & 15u;
+        b4[1] = (hi >> 7) & 15u;
+        b4[2] = (hi >> 3) & 15u;
+        const uint32_t va = (a4[0] << 8) | (a4[1] << 4) | a4[2], vb = (b4[0] << 8) | (b4[1] << 4) | b4[2];
+        di = (((hi >> 2) & 1u) << 2) | ((hi & 1u) << 1) | (va >= vb ? 1u : 0u);
+    }
+    const int32_t d = (int32_t)((0x40292017100B0603ull >> (8 * di)) & 0xFFu);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int32_t a = (int32_t)(a4[c] * 17u), b = (int32_t)(b4[c] * 17u);
+        plane[c] = tmode ? (uint32_t)a | (clamp_u8(b + d) << 8) | ((uint32_t)b << 16) | (clamp_u8(b - d) << 24)
+                         : clamp_u8(a + d) | (clamp_u8(a - d) << 8) | (clamp_u8(b + d) << 16) | (clamp_u8(b - d) << 24);
+    }
+}
+
+// ETC2 planar mode: three RGB676 colours O, H, V (hi, lo = the block's two
+// big-endian words); pixel (x, y) = clamp((x (H - O) + y (V - O) + 4 O + 2) >> 2),
+// alpha 255.  O: red = bits 30-25 of hi, green 24 then 22-17, blue 16 then
+// 12-11 then 9-7; H: red 6-2 then 0, green = bits 31-25 of lo, blue 24-19;
+// V: red 18-13, green 12-6, blue 5-0.  6- and 7-bit values are widened by
+// replicating their top bits.
+__device__ __forceinline__ void etc2_planar(uint32_t hi, uint32_t lo, uint32_t (&px)[16])
+{
+    const uint32_t o6[3] = {(hi >> 25) & 63u, (((hi >> 24) & 1u) << 6) | ((hi >> 17) & 63u),
+                            (((hi >> 16) & 1u) << 5) | (((hi >> 11) & 3u) << 3) | ((hi >> 7) & 7u)};
+    const uint32_t h6[3] = {(((hi >> 2) & 31u) << 1) | (hi & 1u), lo >> 25, (lo >> 19) & 63u};
+    const uint32_t v6[3] = {(lo >> 13) & 63u, (lo >> 6) & 127u, lo & 63u};
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int x = 0; x < 4; ++x) px[4 * y + x] = 0xFF000000u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int up = c == 1 ? 1 : 2, down = c == 1 ? 6 : 4;  // 7-bit green, 6-bit red and blue
+        const int32_t o = (int32_t)((o6[c] << up) | (o6[c] >> down));
+        const int32_t dh = (int32_t)((h6[c] << up) | (h6[c] >> down)) - o;
+        const int32_t dv = (int32_t)((v6[c] << up) | (v6[c] >> down)) - o;
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            int32_t t = 4 * o + 2 + y * dv;
+#pragma unroll
+            for (int x = 0; x < 4; ++x, t += dh) px[4 * y + x] |= clamp_u8(t >> 2) << (8 * c);
+        }
+    }
+}
+
+// 16 decoded pixels (row-major) of an ETC1 / ETC2 colour block; w0/w1 are
+// the block's first/second 4 bytes read as little-endian words.
 //
 // Per channel, the 8 clamped palette bytes of both sub-blocks sit in two
 // words (sub-block 0 in the low word, entry = selector value).  A row's four
 // pixels then come from one v_perm_b32 per channel whose selector bytes are
 // the pixels' 2-bit selectors plus 4 for sub-block 1, and four more
 // v_perm_b32 interleave the channel rows into RGBA words (alpha 255 from
-// selector byte 0x0d).
+// selector byte 0x0d).  T and H blocks use the same lookup with their four
+// paint colours in both words; planar blocks overwrite the pixels after it.
 __device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
 {
     const uint32_t hi = __builtin_bswap32(w0), lo = __builtin_bswap32(w1);  // big-endian words
     const bool diff = (hi >> 1) & 1u, flip = hi & 1u;
     uint32_t base[2][3];
+    uint32_t overflow = 0;  // bit c: channel c's differential sum leaves 0..31
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const uint32_t field = (hi >> (24 - 8 * c)) & 0xFFu;
         if (diff) {
             const uint32_t a5 = field >> 3;
             const int32_t d = (int32_t)(field & 7u) - ((field & 4u) ? 8 : 0);
-            const uint32_t b5 = (uint32_t)((int32_t)a5 + d) & 31u;
+            const uint32_t sum = (uint32_t)((int32_t)a5 + d);
+            overflow |= (sum > 31u ? 1u : 0u) << c;
+            const uint32_t b5 = sum & 31u;
             base[0][c] = (a5 << 3) | (a5 >> 2);
             base[1][c] = (b5 << 3) | (b5 >> 2);
         } else {
@@ -103,9 +183,15 @@ __device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int32_t v = (int32_t)base[s][c];
-            plane[c][s] = (uint32_t)min(max(v + a, 0), 255) | ((uint32_t)min(max(v + b, 0), 255) << 8) |
-                          ((uint32_t)min(max(v - a, 0), 255) << 16) | ((uint32_t)min(max(v - b, 0), 255) << 24);
+            plane[c][s] = clamp_u8(v + a) | (clamp_u8(v + b) << 8) | (clamp_u8(v - a) << 16) | (clamp_u8(v - b) << 24);
         }
+    }
+    // ETC2: red overflow = T mode, else green = H mode (else blue = planar, below)
+    if (__builtin_expect((overflow & 3u) != 0u, 0)) {
+        uint32_t th[3];
+        etc2_th_planes(hi, (overflow & 1u) != 0u, th);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) plane[c][0] = plane[c][1] = th[c];
     }
     // sub-block 1: rows 2..3 (flip) or columns 2..3
 #pragma unroll
@@ -128,6 +214,7 @@ __device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (
         px[4 * y + 2] = __builtin_amdgcn_perm(ba23, rg23, 0x05040100u);
         px[4 * y + 3] = __builtin_amdgcn_perm(ba23, rg23, 0x07060302u);
     }
+    if (__builtin_expect(overflow == 4u, 0)) etc2_planar(hi, lo, px);
 }
 
 // Alpha of four pixels from an 8-entry palette held as bytes of {hi:lo}:

@@ -174,9 +174,10 @@ GOOFY_HIP_API int goofy_hip_tune_device(int format, void* d_result, const void* 
  * colour block in 4-colour mode :945-956, EAC alpha :679-713).  `d_rgba`
  * receives RGBA8 rows of `pitch` bytes (multiple of 16, >= 4*width, 16-byte
  * aligned); `d_blocks` holds (width/4)*(height/4) tightly packed blocks of the
- * format's size, aligned to it.  ETC1 blocks whose differential sums leave
- * 0..31 (ETC2 T/H/planar) are not ETC1 and are decoded with 5-bit
- * wrap-around; DXT1 and ETC1S decode with alpha 255. */
+ * format's size, aligned to it.  Both ETC formats decode every ETC2 colour
+ * mode, as the harness's decoder does (Src/decoder.cpp:716-773): a
+ * differential block whose red, green or blue sum leaves 0..31 is a T, H or
+ * planar block (:270-386).  DXT1 and ETC1S decode with alpha 255. */
 GOOFY_HIP_API int goofy_hip_decode_device(int format, void* d_rgba, const void* d_blocks, unsigned int width,
                                           unsigned int height, size_t pitch, void* stream);
 

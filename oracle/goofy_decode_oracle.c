@@ -6,7 +6,8 @@
  * from the public format definitions (BC1: two RGB565 endpoints, 2-bit indices LSB-first per pixel
  * in row-major order; ETC1: 64-bit big-endian word, individual/differential
  * base colours, eight intensity tables, MSB/LSB selector planes in
- * column-major pixel order).  The rounding of the BC1 interpolants follows
+ * column-major pixel order; ETC2's T, H and planar colour modes, which a
+ * differential sum outside 0..31 selects).  The rounding of the BC1 interpolants follows
  * the decoder the reference's quality harness uses
  * (Src/decoder.cpp:819-871: expand 5/6-bit by bit replication, thirds by
  * truncating division), and the ETC1 tables follow Src/decoder.cpp:388-410.
@@ -62,12 +63,119 @@ static void decode_bc1_mode(const uint8_t* block, uint8_t* out, int four_only)
 
 static const int kEtc1Tables[8][2] = {{2, 8}, {5, 17}, {9, 29}, {13, 42}, {18, 60}, {24, 80}, {33, 106}, {47, 183}};
 
+/* ETC2 T and H modes: four paint colours, pixel (x, y) takes colour
+ * 2*MSB + LSB of its selector bits k = 4x + y (no sign/magnitude mapping).
+ * Both base colours are 4 bits per channel, widened by replication; the
+ * distance table is 3, 6, 11, 16, 23, 32, 41, 64. */
+static const int kEtc2Distance[8] = {3, 6, 11, 16, 23, 32, 41, 64};
+
+static void etc2_paint(uint32_t lo, const int paint[4][3], uint8_t* out)
+{
+    for (int y = 0; y < 4; ++y)
+        for (int x = 0; x < 4; ++x) {
+            const int k = 4 * x + y;
+            const int sel = (int)((((lo >> (16 + k)) & 1u) << 1) | ((lo >> k) & 1u));
+            uint8_t* o = out + 4 * (4 * y + x);
+            for (int c = 0; c < 3; ++c) o[c] = (uint8_t)paint[sel][c];
+            o[3] = 255;
+        }
+}
+
+static int widen4(int v) { return (v << 4) | v; }
+
+/* T mode (the red differential sum leaves 0..31).  Fields of the big-endian
+ * 64-bit block, bit 63 first: colour A red = bits 60-59 (high half) and 57-56,
+ * green 55-52, blue 51-48; colour B red 47-44, green 43-40, blue 39-36;
+ * distance index = bits 35-34 then bit 32.  Paint colours: A, B + d, B, B - d. */
+static void etc2_t_mode(uint32_t hi, uint32_t lo, uint8_t* out)
+{
+    const int a[3] = {widen4((int)((((hi >> 27) & 3u) << 2) | ((hi >> 24) & 3u))), widen4((int)((hi >> 20) & 15u)),
+                      widen4((int)((hi >> 16) & 15u))};
+    const int b[3] = {widen4((int)((hi >> 12) & 15u)), widen4((int)((hi >> 8) & 15u)), widen4((int)((hi >> 4) & 15u))};
+    const int d = kEtc2Distance[(((hi >> 2) & 3u) << 1) | (hi & 1u)];
+    int paint[4][3];
+    for (int c = 0; c < 3; ++c) {
+        paint[0][c] = a[c];
+        paint[1][c] = clamp255(b[c] + d);
+        paint[2][c] = b[c];
+        paint[3][c] = clamp255(b[c] - d);
+    }
+    etc2_paint(lo, paint, out);
+}
+
+/* H mode (red in range, the green sum leaves 0..31).  Colour A red = bits
+ * 62-59, green = 58-56 then 52, blue = 51 then 49-47; colour B red 46-43,
+ * green 42-39, blue 38-35.  Distance index = bit 34, bit 32, then 1 if A >= B
+ * as 12-bit RGB444 numbers.  Paint colours: A + d, A - d, B + d, B - d. */
+static void etc2_h_mode(uint32_t hi, uint32_t lo, uint8_t* out)
+{
+    const int a4[3] = {(int)((hi >> 27) & 15u), (int)((((hi >> 24) & 7u) << 1) | ((hi >> 20) & 1u)),
+                       (int)((((hi >> 19) & 1u) << 3) | ((hi >> 15) & 7u))};
+    const int b4[3] = {(int)((hi >> 11) & 15u), (int)((hi >> 7) & 15u), (int)((hi >> 3) & 15u)};
+    const int va = (a4[0] << 8) | (a4[1] << 4) | a4[2], vb = (b4[0] << 8) | (b4[1] << 4) | b4[2];
+    const int d = kEtc2Distance[(((hi >> 2) & 1u) << 2) | ((hi & 1u) << 1) | (va >= vb ? 1 : 0)];
+    int paint[4][3];
+    for (int c = 0; c < 3; ++c) {
+        paint[0][c] = clamp255(widen4(a4[c]) + d);
+        paint[1][c] = clamp255(widen4(a4[c]) - d);
+        paint[2][c] = clamp255(widen4(b4[c]) + d);
+        paint[3][c] = clamp255(widen4(b4[c]) - d);
+    }
+    etc2_paint(lo, paint, out);
+}
+
+/* Planar mode (red and green in range, the blue sum leaves 0..31): three
+ * RGB676 colours O, H, V; pixel (x, y) = clamp((x (H - O) + y (V - O) + 4 O + 2) >> 2).
+ * O: red 62-57, green 56 then 54-49, blue 48 then 44-43 then 41-39; H: red
+ * 38-34 then 32, green 31-25, blue 24-19; V: red 18-13, green 12-6, blue 5-0. */
+static void etc2_planar(uint32_t hi, uint32_t lo, uint8_t* out)
+{
+    const unsigned o6[3] = {(hi >> 25) & 63u, (((hi >> 24) & 1u) << 6) | ((hi >> 17) & 63u),
+                            (((hi >> 16) & 1u) << 5) | (((hi >> 11) & 3u) << 3) | ((hi >> 7) & 7u)};
+    const unsigned h6[3] = {(((hi >> 2) & 31u) << 1) | (hi & 1u), lo >> 25, (lo >> 19) & 63u};
+    const unsigned v6[3] = {(lo >> 13) & 63u, (lo >> 6) & 127u, lo & 63u};
+    int o[3], h[3], v[3];
+    for (int c = 0; c < 3; ++c) {
+        /* green has 7 bits, red and blue 6; widen by replicating the top bits */
+        const int bits = c == 1 ? 7 : 6;
+        o[c] = (int)((o6[c] << (8 - bits)) | (o6[c] >> (2 * bits - 8)));
+        h[c] = (int)((h6[c] << (8 - bits)) | (h6[c] >> (2 * bits - 8)));
+        v[c] = (int)((v6[c] << (8 - bits)) | (v6[c] >> (2 * bits - 8)));
+    }
+    for (int y = 0; y < 4; ++y)
+        for (int x = 0; x < 4; ++x) {
+            uint8_t* px = out + 4 * (4 * y + x);
+            for (int c = 0; c < 3; ++c) {
+                const int t = x * (h[c] - o[c]) + y * (v[c] - o[c]) + 4 * o[c] + 2;
+                /* floor division by 4 (t may be negative) */
+                px[c] = (uint8_t)clamp255(t >= 0 ? t / 4 : -((-t + 3) / 4));
+            }
+            px[3] = 255;
+        }
+}
+
+/* ETC1 individual and differential modes, and the three modes ETC2 signals
+ * with a differential sum outside 0..31 (red: T, else green: H, else blue:
+ * planar), as the harness's decoder dispatches them (Src/decoder.cpp:716-773). */
 void goofy_oracle_decode_etc1(const uint8_t* block, uint8_t* out)
 {
     const uint32_t hi = ((uint32_t)block[0] << 24) | ((uint32_t)block[1] << 16) | ((uint32_t)block[2] << 8) | block[3];
     const uint32_t lo = ((uint32_t)block[4] << 24) | ((uint32_t)block[5] << 16) | ((uint32_t)block[6] << 8) | block[7];
     const int diff = (int)((hi >> 1) & 1u);
     const int flip = (int)(hi & 1u);
+    if (diff) {
+        for (int c = 0; c < 3; ++c) {
+            const unsigned field = (hi >> (24 - 8 * c)) & 0xFFu;
+            const int d = (int)(field & 7u) - ((field & 4u) ? 8 : 0);
+            const int sum = (int)(field >> 3) + d;
+            if (sum < 0 || sum > 31) {
+                if (c == 0) etc2_t_mode(hi, lo, out);
+                else if (c == 1) etc2_h_mode(hi, lo, out);
+                else etc2_planar(hi, lo, out);
+                return;
+            }
+        }
+    }
     int base[2][3];
     for (int c = 0; c < 3; ++c) {
         const int shift = 24 - 8 * c; /* R at bits 31..24, G 23..16, B 15..8 */
@@ -76,9 +184,9 @@ void goofy_oracle_decode_etc1(const uint8_t* block, uint8_t* out)
             const int b5 = (int)(field >> 3);
             int d = (int)(field & 7u);
             if (d >= 4) d -= 8;
-            const int b5b = b5 + d; /* out-of-range sums are invalid ETC1 */
+            const int b5b = b5 + d; /* 0..31 here: other sums took the ETC2 modes above */
             base[0][c] = (b5 << 3) | (b5 >> 2);
-            base[1][c] = ((b5b & 31) << 3) | ((b5b & 31) >> 2);
+            base[1][c] = (b5b << 3) | (b5b >> 2);
         } else {
             const int a4 = (int)(field >> 4), b4 = (int)(field & 15u);
             base[0][c] = (a4 << 4) | a4;

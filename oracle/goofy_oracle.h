@@ -58,13 +58,14 @@ int goofy_oracle_compress_blocks(int fmt, uint8_t* dst, const uint8_t* src, unsi
 int goofy_oracle_compress_mt(int fmt, uint8_t* dst, const uint8_t* src, unsigned width,
                              unsigned height, size_t stride, int nthreads);
 
-/* Independent decoders (BC1 4-colour/3-colour, ETC1 individual and
- * differential modes).  out = 16 RGBA8 pixels, row-major. */
+/* Independent decoders (BC1 4-colour/3-colour; ETC1 individual and
+ * differential modes and ETC2's T, H and planar colour modes).  out = 16
+ * RGBA8 pixels, row-major. */
 void goofy_oracle_decode_bc1(const uint8_t* block, uint8_t* out);
 void goofy_oracle_decode_etc1(const uint8_t* block, uint8_t* out);
 /* 16-byte blocks: BC3 (alpha block, then a colour block always decoded in
- * 4-colour mode) and ETC2 RGBA8 (EAC alpha block, then an ETC1-compatible
- * colour block: individual and differential modes without overflow). */
+ * 4-colour mode) and ETC2 RGBA8 (EAC alpha block, then an ETC2 colour
+ * block in any of its five modes). */
 void goofy_oracle_decode_bc3(const uint8_t* block, uint8_t* out);
 void goofy_oracle_decode_etc2(const uint8_t* block, uint8_t* out);
 /* Whole images, tightly packed block rows; dst pitch = width*4. */

@@ -17,18 +17,44 @@ from goofy_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
+def etc_modes(blocks):
+    """Colour mode of each 8-byte ETC block: 0 individual, 1 differential,
+    2 T, 3 H, 4 planar (ETC2's modes, signalled by a differential red, green
+    or blue sum outside 0..31, checked in that order)."""
+    b = blocks.astype(np.int32)
+    diff = (b[:, 3] & 2) != 0
+    over = [((b[:, c] >> 3) + ((b[:, c] & 7) - 8 * ((b[:, c] & 4) != 0)) < 0) |
+            ((b[:, c] >> 3) + ((b[:, c] & 7) - 8 * ((b[:, c] & 4) != 0)) > 31) for c in range(3)]
+    return np.select([~diff, over[0], over[1], over[2]], [0, 2, 3, 4], 1)
+
+
 def random_etc1(rng, n):
-    out = []
-    while len(out) < n:
-        b = rng.integers(0, 256, 8, dtype=np.uint8)
-        if b[3] & 2 and not all(0 <= (int(v) >> 3) + ((int(v) & 7) - 8 * ((int(v) & 7) >= 4)) <= 31 for v in b[:3]):
-            continue
-        out.append(b)
-    return np.stack(out)
+    """Random 8-byte ETC colour blocks, every mode included.  Random bytes
+    give about 3% each of T, H and planar blocks; 30% more are forced into
+    one of the three by setting the differential bit, keeping the channels
+    before the chosen one in range (base 4..27) and making the chosen one
+    overflow (base 31 with delta +1..+3, or base 0 with delta -3..-1)."""
+    b = rng.integers(0, 256, (n, 8), dtype=np.uint8)
+    forced = np.nonzero(rng.random(n) < 0.3)[0]
+    ch = rng.integers(0, 3, forced.size)
+    low = rng.random(forced.size) < 0.5
+    d3 = rng.integers(1, 4, forced.size)
+    b[forced, 3] |= 2
+    for j, i in enumerate(forced):
+        for c in range(ch[j]):
+            b[i, c] = (int(b[i, c]) & 7) | ((4 + int(b[i, c]) % 24) << 3)
+        b[i, ch[j]] = (8 - d3[j]) if low[j] else ((31 << 3) | d3[j])
+    modes = etc_modes(b)
+    for m in (2, 3, 4):
+        assert (modes == m).mean() > 0.08, (m, (modes == m).mean())
+    return b
 
 
 @pytest.mark.parametrize("fmt", [W.DXT1, W.ETC1])
-def test_decode_matches_oracle(gpu, oracle, fmt):
+def test_decode_matches_oracle(gpu, oracle, harness, fmt):
+    """Random blocks: every BC1 mode, and every ETC colour mode (individual,
+    differential, T, H, planar), against the oracle's and the harness's
+    decoders."""
     from goofy_amd import hip
     rng = np.random.default_rng(7 + fmt)
     bw, bh = 100, 37
@@ -44,14 +70,15 @@ def test_decode_matches_oracle(gpu, oracle, fmt):
     got = d_img.download().reshape(h, pitch)
     want = oracle.decode_image(fmt, blocks.reshape(-1), w, h).reshape(h, w * 4)
     assert np.array_equal(got[:, : w * 4], want)
+    assert np.array_equal(want.reshape(-1), harness.decompress(fmt, blocks.reshape(-1), w, h))
     assert (got[:, w * 4:] == 0x3C).all()  # padding untouched
 
 
 @pytest.mark.parametrize("fmt", [W.DXT5, W.ETC2])
 def test_decode_alpha_formats_match_oracle_and_harness(gpu, oracle, harness, fmt):
     """Random alpha blocks (every BC3 mode, every EAC table and multiplier,
-    0 included) with valid colour halves, against the oracle's and the
-    harness's decoders."""
+    0 included) with random colour halves (every ETC2 colour mode), against
+    the oracle's and the harness's decoders."""
     from goofy_amd import hip
     rng = np.random.default_rng(17 + fmt)
     bw, bh = 96, 40
@@ -130,6 +157,30 @@ def test_quality_alpha_formats_match_harness(gpu, harness, fmt):
         assert q[k] == pytest.approx(ref[k], rel=1e-12), k
     assert q["mse_y"] == pytest.approx(ref["mse_y"], rel=1e-9)
     assert q["psnr_a"] > 40.0  # nearest-level alpha: error at most a few units per pixel
+
+
+@pytest.mark.parametrize("fmt", [W.ETC1, W.ETC2])
+def test_quality_of_foreign_etc_blocks(gpu, harness, fmt):
+    """The round-trip sums over ETC blocks the encoder never writes (random
+    colour halves in all five modes, random EAC alpha): the GPU quality call
+    against getMsePsnr of the harness over the harness's own decode."""
+    from goofy_amd import hip
+    rng = np.random.default_rng(70 + fmt)
+    bw, bh = 64, 48
+    n = bw * bh
+    colour = random_etc1(rng, n)
+    blocks = colour if fmt == W.ETC1 else np.concatenate([rng.integers(0, 256, (n, 8), dtype=np.uint8), colour], 1)
+    w, h = 4 * bw, 4 * bh
+    img = rng.integers(0, 256, w * h * 4, dtype=np.uint8)
+    print(f"quality of foreign ETC blocks fmt={fmt} {w}x{h}", flush=True)
+    q = gpu.quality_device(fmt, hip.DeviceBuffer.from_array(blocks.reshape(-1)), hip.DeviceBuffer.from_array(img),
+                           w, h, w * 4)
+    dec = harness.decompress(fmt, blocks.reshape(-1), w, h)
+    ref = harness.mse_psnr(img, dec, w, h)
+    assert q["sse_byte"] == reference_quality(img, dec)["sse_byte"]
+    for k in ("mse_r", "mse_g", "mse_b", "mse_a", "mse_rgb", "psnr_min", "psnr_rgb"):
+        assert q[k] == pytest.approx(ref[k], rel=1e-12), k
+    assert q["mse_y"] == pytest.approx(ref["mse_y"], rel=1e-9)
 
 
 def test_quality_of_black_image_closed_form(gpu):
