@@ -426,6 +426,8 @@ const char* goofy_hip_status_string(int s)
     case GOOFY_ERR_NO_DEVICE: return "no GPU";
     case GOOFY_ERR_TOO_LARGE: return "image too large";
     case GOOFY_ERR_BUSY: return "busy";
+    case GOOFY_ERR_CANCELLED: return "cancelled";
+    case GOOFY_ERR_DEADLINE: return "deadline passed";
     default: return "unknown status";
     }
 }

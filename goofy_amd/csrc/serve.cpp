@@ -5,16 +5,18 @@
 // in HBM and turns a stream of independent requests into overlapping batch
 // launches:
 //
-//   submit() --> incoming queue --> [scheduler thread]
+//   submit() --> pending queue --> [scheduler thread]
 //                                     | admit in arrival order while the
 //                                     | in-flight byte budget and the output
-//                                     | arena allow
+//                                     | arena allow; complete expired
+//                                     | requests unlaunched
 //                                     v
 //                 one batch-kernel launch per admitted group, on a free
 //                 stream (up to `streams` groups in flight at once)
 //                                     |
 //                 event polled done --> completion queue --> poll()
 //                 release(ticket) frees the output range and budget share
+//   cancel(ticket) takes a request out of the pending queue.
 //
 // Requests that arrive while groups are running are admitted and launched
 // immediately on another stream, so their work overlaps on the device with
@@ -107,7 +109,7 @@ struct Req {
     uint64_t ticket = 0;
     goofy_request r{};
     size_t in_bytes = 0, out_bytes = 0, arena_off = SIZE_MAX;
-    double t_submit = 0, t_launch = 0;
+    double t_arrival = 0, t_submit = 0, t_launch = 0;
 };
 
 struct Slot {
@@ -130,9 +132,9 @@ struct goofy_server {
     Arena arena;
     std::vector<Slot> slots;
 
-    std::mutex mu;  // incoming, completions, budget, arena, outstanding
+    std::mutex mu;  // everything below up to `stop`
     std::condition_variable cv_work, cv_idle;
-    std::deque<Req> incoming;
+    std::deque<Req> pending;  // submitted, not yet admitted, in arrival order
     std::deque<goofy_completion> completions;
     struct Held {
         size_t off, bytes;
@@ -142,9 +144,13 @@ struct goofy_server {
     uint64_t next_ticket = 1;
     size_t budget_used = 0, peak_used = 0;
     goofy_server_stats st{};
-    uint64_t unfinished = 0;  // submitted, not yet completed
+    uint64_t unfinished = 0;     // submitted, not yet completed
+    uint64_t inflight_reqs = 0;  // admitted, not yet completed
+    double min_deadline = 0;     // earliest deadline among pending requests (0 = none)
+    // Nothing runs and the oldest pending request waits for budget or arena
+    // space held by completed, unreleased outputs: only releases can help.
+    bool stalled = false;
 
-    std::deque<Req> pending;  // scheduler thread only
     std::atomic<bool> stop{false};
     std::thread thread;
 
@@ -152,7 +158,42 @@ struct goofy_server {
     void run();
     bool launch_group(Slot& s, std::vector<Req>& group);
     void finish(Slot& s, int status);
+    void complete_unlaunched(const Req& q, int status, double t);  // caller holds mu
+    void expire_pending(double t);                                  // caller holds mu
 };
+
+void goofy_server::complete_unlaunched(const Req& q, int status, double t)
+{
+    goofy_completion c{};
+    c.ticket = q.ticket;
+    c.user_tag = q.r.user_tag;
+    c.status = status;
+    c.t_arrival_us = q.t_arrival;
+    c.t_submit_us = q.t_submit;
+    c.t_done_us = t;
+    completions.push_back(c);
+    ++st.completed;
+    if (status == GOOFY_ERR_CANCELLED) ++st.cancelled;
+    else ++st.expired;
+    if (--unfinished == 0) cv_idle.notify_all();
+}
+
+void goofy_server::expire_pending(double t)
+{
+    if (min_deadline == 0 || t < min_deadline) return;
+    double next = 0;
+    for (auto it = pending.begin(); it != pending.end();) {
+        const double d = it->r.deadline_us;
+        if (d > 0 && d <= t) {
+            complete_unlaunched(*it, GOOFY_ERR_DEADLINE, t);
+            it = pending.erase(it);
+        } else {
+            if (d > 0 && (next == 0 || d < next)) next = d;
+            ++it;
+        }
+    }
+    min_deadline = next;
+}
 
 void goofy_server::finish(Slot& s, int status)
 {
@@ -167,12 +208,14 @@ void goofy_server::finish(Slot& s, int status)
         c.status = status;
         c.dst = arena.base() + q.arena_off;
         c.dst_bytes = q.out_bytes;
+        c.t_arrival_us = q.t_arrival;
         c.t_submit_us = q.t_submit;
         c.t_launch_us = q.t_launch;
         c.t_done_us = t;
         completions.push_back(c);
         ++st.completed;
         --unfinished;
+        --inflight_reqs;
     }
     s.reqs.clear();
     s.busy = false;
@@ -238,32 +281,11 @@ bool goofy_server::launch_group(Slot& s, std::vector<Req>& group)
     {
         std::lock_guard<std::mutex> lock(mu);
         st.launches += cfg.batching ? 1 : s.reqs.size();  // kernel launches
-        st.admitted += s.reqs.size();
     }
     if (e != hipSuccess) {
         // report the failure on every request of the group
         (void)hipStreamSynchronize(s.stream);
-        const double tf = now_us();
-        std::lock_guard<std::mutex> lock(mu);
-        for (Req& q : s.reqs) {
-            goofy_completion c{};
-            c.ticket = q.ticket;
-            c.user_tag = q.r.user_tag;
-            c.status = GOOFY_ERR_HIP;
-            c.dst = arena.base() + q.arena_off;
-            c.dst_bytes = q.out_bytes;
-            c.t_submit_us = q.t_submit;
-            c.t_launch_us = q.t_launch;
-            c.t_done_us = tf;
-            budget_used -= q.in_bytes;
-            outstanding[q.ticket].done = true;
-            completions.push_back(c);
-            ++st.completed;
-            --unfinished;
-        }
-        s.reqs.clear();
-        s.busy = false;
-        if (unfinished == 0) cv_idle.notify_all();
+        finish(s, GOOFY_ERR_HIP);
         return false;
     }
     return true;
@@ -287,26 +309,20 @@ void goofy_server::run()
             finish(s, q == hipSuccess ? GOOFY_OK : GOOFY_ERR_HIP);
             progress = true;
         }
-        {
-            std::lock_guard<std::mutex> lock(mu);
-            while (!incoming.empty()) {
-                pending.push_back(std::move(incoming.front()));
-                incoming.pop_front();
-            }
-        }
         // admission: arrival order, stop at the first request that does not fit
-        while (!pending.empty()) {
+        while (true) {
             Slot* free_slot = nullptr;
-            for (Slot& s : slots)
-                if (!s.busy) {
-                    free_slot = &s;
-                    break;
-                }
-            if (!free_slot) break;
+            bool any_busy = false;
+            for (Slot& s : slots) {
+                if (!s.busy && !free_slot) free_slot = &s;
+                any_busy |= s.busy;
+            }
             group.clear();
             {
                 std::lock_guard<std::mutex> lock(mu);
-                while (!pending.empty() && (int)group.size() < max_batch) {
+                const double t = now_us();
+                expire_pending(t);
+                while (free_slot && !pending.empty() && (int)group.size() < max_batch) {
                     Req& q = pending.front();
                     const size_t need = q.in_bytes + align_up(q.out_bytes);
                     if (budget_used + need > cfg.hbm_budget_bytes) break;
@@ -319,6 +335,13 @@ void goofy_server::run()
                     group.push_back(std::move(q));
                     pending.pop_front();
                 }
+                st.admitted += group.size();
+                inflight_reqs += group.size();
+                // nothing runs, yet the head does not fit: the budget or the
+                // arena is held by completed outputs that await release
+                const bool now_stalled = group.empty() && free_slot && !any_busy && !pending.empty();
+                if (now_stalled && !stalled) cv_idle.notify_all();
+                stalled = now_stalled;
             }
             if (group.empty()) break;
             launch_group(*free_slot, group);
@@ -332,14 +355,16 @@ void goofy_server::run()
         for (Slot& s : slots) any_busy |= s.busy;
         const double t = now_us();
         if (any_busy || idle_since < 0 || t - idle_since < kSpinUs) {
-            // poll events / the incoming queue again; spinning a little before
+            // poll events / the pending queue again; spinning a little before
             // sleeping keeps wake-up latency off the request path
             if (!any_busy && idle_since < 0) idle_since = t;
             std::this_thread::yield();
         } else {
+            // woken by submits, releases and cancels; the timeout bounds how
+            // late an expired deadline is reported
             std::unique_lock<std::mutex> lock(mu);
             cv_work.wait_for(lock, std::chrono::microseconds(500), [&] {
-                return !incoming.empty() || stop.load(std::memory_order_acquire);
+                return (!pending.empty() && !stalled) || stop.load(std::memory_order_acquire);
             });
             idle_since = -1;
         }
@@ -433,15 +458,18 @@ int goofy_server_submit_many(goofy_server* s, const goofy_request* reqs, int n, 
         for (int i = 0; i < n; ++i) {
             Req q;
             if ((rc = validate_request(s, &reqs[i], &q)) != GOOFY_OK) break;
-            if (s->st.submitted - s->st.admitted >= (uint64_t)s->cfg.max_pending) {
+            if (s->pending.size() >= (size_t)s->cfg.max_pending) {
                 ++s->st.rejected;
                 rc = fail(GOOFY_ERR_BUSY, "pending queue full");
                 break;
             }
             q.ticket = s->next_ticket++;
             q.t_submit = t;
+            q.t_arrival = q.r.arrival_us > 0 ? q.r.arrival_us : t;
+            const double d = q.r.deadline_us;
+            if (d > 0 && (s->min_deadline == 0 || d < s->min_deadline)) s->min_deadline = d;
             if (tickets) tickets[i] = q.ticket;
-            s->incoming.push_back(q);
+            s->pending.push_back(q);
             ++s->st.submitted;
             ++s->unfinished;
             ++*accepted;
@@ -482,6 +510,7 @@ static int release_locked(goofy_server* s, uint64_t ticket)
     s->budget_used -= align_up(it->second.bytes);
     s->outstanding.erase(it);
     ++s->st.released;
+    s->stalled = false;  // the scheduler looks at the pending queue again
     return GOOFY_OK;
 }
 
@@ -489,8 +518,13 @@ int goofy_server_release(goofy_server* s, uint64_t ticket)
 {
     clear_error();
     if (!s) return fail(GOOFY_ERR_INVALID, "null server");
-    std::lock_guard<std::mutex> lock(s->mu);
-    return release_locked(s, ticket);
+    int rc;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        rc = release_locked(s, ticket);
+    }
+    if (rc == GOOFY_OK) s->cv_work.notify_one();
+    return rc;
 }
 
 int goofy_server_release_many(goofy_server* s, const uint64_t* tickets, int n)
@@ -498,12 +532,33 @@ int goofy_server_release_many(goofy_server* s, const uint64_t* tickets, int n)
     clear_error();
     if (!s || (n > 0 && !tickets)) return fail(GOOFY_ERR_INVALID, "null argument");
     int rc = GOOFY_OK;
-    std::lock_guard<std::mutex> lock(s->mu);
-    for (int i = 0; i < n; ++i) {
-        const int r = release_locked(s, tickets[i]);
-        if (r != GOOFY_OK && rc == GOOFY_OK) rc = r;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        for (int i = 0; i < n; ++i) {
+            const int r = release_locked(s, tickets[i]);
+            if (r != GOOFY_OK && rc == GOOFY_OK) rc = r;
+        }
     }
+    if (n > 0) s->cv_work.notify_one();
     return rc;
+}
+
+int goofy_server_cancel(goofy_server* s, uint64_t ticket)
+{
+    clear_error();
+    if (!s) return fail(GOOFY_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    // newest first: a caller usually cancels what it submitted last
+    for (auto it = s->pending.rbegin(); it != s->pending.rend(); ++it) {
+        if (it->ticket != ticket) continue;
+        s->complete_unlaunched(*it, GOOFY_ERR_CANCELLED, s->now_us());
+        s->pending.erase(std::next(it).base());
+        s->stalled = false;  // a new head may fit
+        s->cv_work.notify_one();
+        return GOOFY_OK;
+    }
+    if (s->outstanding.count(ticket)) return fail(GOOFY_ERR_BUSY, "request already launched");
+    return fail(GOOFY_ERR_INVALID, "unknown, completed or already cancelled ticket");
 }
 
 int goofy_server_wait_idle(goofy_server* s, int timeout_ms)
@@ -511,10 +566,12 @@ int goofy_server_wait_idle(goofy_server* s, int timeout_ms)
     clear_error();
     if (!s) return fail(GOOFY_ERR_INVALID, "null server");
     std::unique_lock<std::mutex> lock(s->mu);
-    auto idle = [&] { return s->unfinished == 0; };
-    if (timeout_ms < 0) s->cv_idle.wait(lock, idle);
-    else if (!s->cv_idle.wait_for(lock, std::chrono::milliseconds(timeout_ms), idle))
+    auto done_or_stuck = [&] { return s->unfinished == 0 || s->stalled; };
+    if (timeout_ms < 0) s->cv_idle.wait(lock, done_or_stuck);
+    else if (!s->cv_idle.wait_for(lock, std::chrono::milliseconds(timeout_ms), done_or_stuck))
         return fail(GOOFY_ERR_BUSY, "requests still in flight");
+    if (s->unfinished != 0)
+        return fail(GOOFY_ERR_BUSY, "pending requests wait for completed outputs to be released");
     return GOOFY_OK;
 }
 
@@ -524,8 +581,8 @@ int goofy_server_stats_get(goofy_server* s, goofy_server_stats* out)
     if (!s || !out) return fail(GOOFY_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> lock(s->mu);
     *out = s->st;
-    out->pending = s->st.submitted - s->st.admitted;
-    out->inflight_requests = s->st.admitted - s->st.completed;
+    out->pending = s->pending.size();
+    out->inflight_requests = s->inflight_reqs;
     out->inflight_bytes = s->budget_used;
     out->peak_inflight_bytes = s->peak_used;
     return GOOFY_OK;
@@ -577,7 +634,7 @@ int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_resu
     };
     auto next_unit = [&]() { return ((next_u64() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
 
-    std::vector<double> lat, queue;
+    std::vector<double> lat, queue, lag, lat_submit;
     std::vector<uint8_t> host;
     std::atomic<bool> gen_done{false};
     std::atomic<uint64_t> outstanding{0};
@@ -597,8 +654,10 @@ int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_resu
             done.clear();
             for (int i = 0; i < n; ++i) {
                 const goofy_completion& c = buf[i];
-                lat.push_back(c.t_done_us - c.t_submit_us);
-                queue.push_back(c.t_launch_us - c.t_submit_us);
+                lat.push_back(c.t_done_us - c.t_arrival_us);
+                queue.push_back(c.t_launch_us - c.t_arrival_us);
+                lag.push_back(c.t_submit_us - c.t_arrival_us);
+                lat_submit.push_back(c.t_done_us - c.t_submit_us);
                 const goofy_texture_desc& t = cfg->pool[c.user_tag];
                 mpix += (double)t.width * t.height / 1e6;
                 if (c.status != GOOFY_OK) ++mismatches;
@@ -639,15 +698,21 @@ int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_resu
         if (now >= t_end) break;
         chunk.clear();
         if (cfg->rate_per_s > 0) {
-            // every arrival that is due by now, in one call
+            // every arrival that is due by now, in one call, each stamped with
+            // its scheduled arrival time
             while (next_arrival <= now && chunk.size() < 4096) {
                 const int k = (int)(next_u64() % (uint64_t)cfg->pool_count);
                 const goofy_texture_desc& t = cfg->pool[k];
-                chunk.push_back(goofy_request{t.src, t.width, t.height, t.pitch, t.format, (uint64_t)k});
+                chunk.push_back(goofy_request{t.src, t.width, t.height, t.pitch, t.format, (uint64_t)k, next_arrival,
+                                              0.0});
                 next_arrival += -std::log(next_unit()) / cfg->rate_per_s * 1e6;
             }
             if (chunk.empty()) {
-                if (next_arrival - now > 50.0) std::this_thread::sleep_for(std::chrono::microseconds(20));
+                // sleep only through long gaps: a sleep can overrun by the
+                // timer slack (about 50 us), shorter waits spin
+                const double gap = next_arrival - now;
+                if (gap > 300.0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(gap - 200.0)));
+                else std::this_thread::yield();
                 continue;
             }
         } else {
@@ -660,7 +725,7 @@ int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_resu
             for (uint64_t j = 0; j < room; ++j) {
                 const int k = (int)(next_u64() % (uint64_t)cfg->pool_count);
                 const goofy_texture_desc& t = cfg->pool[k];
-                chunk.push_back(goofy_request{t.src, t.width, t.height, t.pitch, t.format, (uint64_t)k});
+                chunk.push_back(goofy_request{t.src, t.width, t.height, t.pitch, t.format, (uint64_t)k, 0.0, 0.0});
             }
         }
         tickets.resize(chunk.size());
@@ -691,6 +756,8 @@ int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_resu
     std::vector<double> sl = lat, sq = queue;
     std::sort(sl.begin(), sl.end());
     std::sort(sq.begin(), sq.end());
+    std::sort(lag.begin(), lag.end());
+    std::sort(lat_submit.begin(), lat_submit.end());
     auto pct = [](const std::vector<double>& v, double p) {
         if (v.empty()) return 0.0;
         const size_t i = std::min(v.size() - 1, (size_t)std::floor(p * (double)(v.size() - 1) + 0.5));
@@ -717,6 +784,11 @@ int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_resu
     res->queue_p99_us = pct(sq, 0.99);
     res->mean_textures_per_launch = st.launches ? (double)st.admitted / (double)st.launches : 0;
     res->peak_inflight_bytes = st.peak_inflight_bytes;
+    res->gen_lag_p50_us = pct(lag, 0.50);
+    res->gen_lag_p99_us = pct(lag, 0.99);
+    res->gen_lag_max_us = lag.empty() ? 0 : lag.back();
+    res->submit_lat_p50_us = pct(lat_submit, 0.50);
+    res->submit_lat_p99_us = pct(lat_submit, 0.99);
     return GOOFY_OK;
 }
 

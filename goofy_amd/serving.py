@@ -26,19 +26,20 @@ class ServerConfig(ctypes.Structure):
 
 class Request(ctypes.Structure):
     _fields_ = [("src", _vp), ("width", ctypes.c_uint), ("height", ctypes.c_uint), ("pitch", ctypes.c_size_t),
-                ("format", ctypes.c_int), ("user_tag", _u64)]
+                ("format", ctypes.c_int), ("user_tag", _u64), ("arrival_us", ctypes.c_double),
+                ("deadline_us", ctypes.c_double)]
 
 
 class Completion(ctypes.Structure):
     _fields_ = [("ticket", _u64), ("user_tag", _u64), ("status", ctypes.c_int), ("dst", _vp),
-                ("dst_bytes", ctypes.c_size_t), ("t_submit_us", ctypes.c_double),
+                ("dst_bytes", ctypes.c_size_t), ("t_arrival_us", ctypes.c_double), ("t_submit_us", ctypes.c_double),
                 ("t_launch_us", ctypes.c_double), ("t_done_us", ctypes.c_double)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, _u64) for n in ("submitted", "admitted", "completed", "released", "rejected", "launches",
                                     "pending", "inflight_requests", "inflight_bytes", "peak_inflight_bytes",
-                                    "budget_bytes")]
+                                    "budget_bytes", "cancelled", "expired")]
 
 
 class SimConfig(ctypes.Structure):
@@ -53,7 +54,9 @@ class SimResult(ctypes.Structure):
                 [(n, ctypes.c_double) for n in ("wall_s", "megapixels", "mps", "offered_mps", "lat_mean_us",
                                                 "lat_p50_us", "lat_p90_us", "lat_p99_us", "lat_max_us",
                                                 "queue_p50_us", "queue_p99_us", "mean_textures_per_launch")] +
-                [("peak_inflight_bytes", _u64)])
+                [("peak_inflight_bytes", _u64)] +
+                [(n, ctypes.c_double) for n in ("gen_lag_p50_us", "gen_lag_p99_us", "gen_lag_max_us",
+                                                "submit_lat_p50_us", "submit_lat_p99_us")])
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -63,6 +66,7 @@ _lib.goofy_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTE
 _lib.goofy_server_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(_u64)]
 _lib.goofy_server_poll.argtypes = [_vp, ctypes.POINTER(Completion), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
 _lib.goofy_server_release.argtypes = [_vp, _u64]
+_lib.goofy_server_cancel.argtypes = [_vp, _u64]
 _lib.goofy_server_wait_idle.argtypes = [_vp, ctypes.c_int]
 _lib.goofy_server_stats_get.argtypes = [_vp, ctypes.POINTER(ServerStats)]
 _lib.goofy_server_now_us.argtypes = [_vp]
@@ -88,8 +92,11 @@ class Server:
         _check(_lib.goofy_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
 
-    def submit(self, src, width: int, height: int, fmt: int, pitch: int | None = None, tag: int = 0) -> int:
-        r = Request(_ptr(src), width, height, width * 4 if pitch is None else pitch, fmt, tag)
+    def submit(self, src, width: int, height: int, fmt: int, pitch: int | None = None, tag: int = 0,
+               arrival_us: float = 0.0, deadline_us: float = 0.0) -> int:
+        """Submit one request; arrival_us / deadline_us are on the server clock (now_us())."""
+        r = Request(_ptr(src), width, height, width * 4 if pitch is None else pitch, fmt, tag, arrival_us,
+                    deadline_us)
         t = _u64()
         _check(_lib.goofy_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
         return int(t.value)
@@ -99,6 +106,13 @@ class Server:
         r = Request(_ptr(src), width, height, width * 4 if pitch is None else pitch, fmt, tag)
         t = _u64()
         return int(_lib.goofy_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
+
+    def cancel(self, ticket: int) -> int:
+        """Cancel an unlaunched request (status code: 0, or -8 when already launched)."""
+        return int(_lib.goofy_server_cancel(self._h, ticket))
+
+    def now_us(self) -> float:
+        return float(_lib.goofy_server_now_us(self._h))
 
     def poll(self, max_items: int = 256) -> list[Completion]:
         buf = (Completion * max_items)()
@@ -196,8 +210,12 @@ def bench_serving(g, hip, W, log=print, rate: float = 20000.0, duration_s: float
                     f"textures (W,H log-uniform multiples of 4 in [64,4096], mean {mean_mpx:.2f} Mpx), "
                     "50/50 DXT1/ETC1S, budget 32 GiB, 4 streams",
         "requests": int(fixed["completed"]), "offered_mps": r(fixed["offered_mps"]), "mps": r(fixed["mps"]),
+        "latency_from": "scheduled Poisson arrival (includes the generator's lag)",
         "latency_us": {"p50": r(fixed["lat_p50_us"]), "p90": r(fixed["lat_p90_us"]), "p99": r(fixed["lat_p99_us"]),
                        "mean": r(fixed["lat_mean_us"]), "max": r(fixed["lat_max_us"])},
+        "generator_lag_us": {"p50": r(fixed["gen_lag_p50_us"]), "p99": r(fixed["gen_lag_p99_us"]),
+                             "max": r(fixed["gen_lag_max_us"])},
+        "submit_to_done_us": {"p50": r(fixed["submit_lat_p50_us"]), "p99": r(fixed["submit_lat_p99_us"])},
         "queue_us": {"p50": r(fixed["queue_p50_us"]), "p99": r(fixed["queue_p99_us"])},
         "textures_per_launch": r(fixed["mean_textures_per_launch"], 2),
         "verified_requests": int(fixed["verified"]), "mismatches": int(fixed["mismatches"]),

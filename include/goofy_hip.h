@@ -51,7 +51,11 @@ typedef enum goofy_status {
     GOOFY_ERR_TOO_LARGE = -7,      /* rows of 4,194,304 pixels or more (larger images of narrower rows run
                                       as horizontal stripes); probe and serving: blocks * blocks-per-row
                                       >= 2^40 */
-    GOOFY_ERR_BUSY = -8            /* serving: admission refused (budget or queue full) */
+    GOOFY_ERR_BUSY = -8,           /* serving: admission refused (queue full), request already launched
+                                      (cancel), or pending requests wait for outputs to be released (wait_idle) */
+    GOOFY_ERR_CANCELLED = -9,      /* serving: completion status of a request cancelled before its launch */
+    GOOFY_ERR_DEADLINE = -10       /* serving: completion status of a request whose deadline passed before its
+                                      launch (it was never launched) */
 } goofy_status;
 
 /* Message for the calling thread's most recent failure ("" if none). */
@@ -303,16 +307,25 @@ typedef struct goofy_request {
     size_t pitch;        /* >= 4*width, multiple of 16 */
     int format;          /* goofy_format */
     uint64_t user_tag;
+    double arrival_us;   /* server clock (goofy_server_now_us) when the request arrived at the caller,
+                            0 = its submit time; completions report it, and latency is measured from it */
+    double deadline_us;  /* server clock; still unlaunched then, the request completes with
+                            GOOFY_ERR_DEADLINE and is never launched (0 = no deadline) */
 } goofy_request;
 
+/* Completions with status GOOFY_OK or GOOFY_ERR_HIP hold an output range (and
+ * its budget share) until goofy_server_release; GOOFY_ERR_CANCELLED and
+ * GOOFY_ERR_DEADLINE completions never held one (dst NULL) and take no
+ * release. */
 typedef struct goofy_completion {
     uint64_t ticket;
     uint64_t user_tag;
     int status;          /* GOOFY_OK or an error code */
     void* dst;           /* device address of the blocks (valid until released) */
     size_t dst_bytes;
-    double t_submit_us;  /* server clock, microseconds */
-    double t_launch_us;
+    double t_arrival_us; /* server clock, microseconds: the request's arrival_us (or its submit time) */
+    double t_submit_us;
+    double t_launch_us;  /* 0 when never launched */
     double t_done_us;    /* completion observed by the scheduler */
 } goofy_completion;
 
@@ -320,6 +333,7 @@ typedef struct goofy_server_stats {
     uint64_t submitted, admitted, completed, released, rejected;
     uint64_t launches, pending, inflight_requests;
     uint64_t inflight_bytes, peak_inflight_bytes, budget_bytes;
+    uint64_t cancelled, expired; /* completed with GOOFY_ERR_CANCELLED / GOOFY_ERR_DEADLINE */
 } goofy_server_stats;
 
 typedef struct goofy_server goofy_server;
@@ -339,7 +353,14 @@ GOOFY_HIP_API int goofy_server_poll(goofy_server* server, goofy_completion* out,
 GOOFY_HIP_API int goofy_server_release(goofy_server* server, uint64_t ticket);
 /* Releases many tickets under one lock; returns the first error, if any. */
 GOOFY_HIP_API int goofy_server_release_many(goofy_server* server, const uint64_t* tickets, int count);
-/* Waits until every submitted request has completed (timeout_ms < 0: forever). */
+/* Cancels a request that has not been launched: it completes at once with
+ * GOOFY_ERR_CANCELLED.  GOOFY_ERR_BUSY when it was already launched (it runs
+ * to completion), GOOFY_ERR_INVALID for an unknown or finished ticket. */
+GOOFY_HIP_API int goofy_server_cancel(goofy_server* server, uint64_t ticket);
+/* Waits until every submitted request has completed (timeout_ms < 0: no
+ * limit).  GOOFY_ERR_BUSY at the timeout, or as soon as nothing runs and the
+ * oldest pending request waits for budget or arena space that only releases
+ * of completed outputs can give back (poll and release, then wait again). */
 GOOFY_HIP_API int goofy_server_wait_idle(goofy_server* server, int timeout_ms);
 GOOFY_HIP_API int goofy_server_stats_get(goofy_server* server, goofy_server_stats* stats);
 GOOFY_HIP_API double goofy_server_now_us(const goofy_server* server);
@@ -348,9 +369,11 @@ GOOFY_HIP_API int goofy_server_destroy(goofy_server* server);
 /* Load generator around a server: Poisson arrivals at `rate_per_s` (0 =
  * closed loop: keep `closed_loop_depth` requests outstanding) for
  * `duration_s`, each request one texture drawn uniformly from `pool`
- * (device-resident; dst fields ignored).  Completed outputs are released at
- * once; every `verify_every`-th is first copied back and compared with
- * `pool_expected[i]` (host bytes of that texture encoded alone; NULL = no check). */
+ * (device-resident; dst fields ignored) and stamped with its scheduled
+ * arrival time, so that latency includes the generator's own lag.  Completed
+ * outputs are released at once; every `verify_every`-th is first copied back
+ * and compared with `pool_expected[i]` (host bytes of that texture encoded
+ * alone; NULL = no check). */
 typedef struct goofy_serve_sim_config {
     double rate_per_s;
     double duration_s;
@@ -366,10 +389,12 @@ typedef struct goofy_serve_sim_config {
 typedef struct goofy_serve_sim_result {
     uint64_t submitted, completed, rejected, verified, mismatches, launches;
     double wall_s, megapixels, mps, offered_mps;
-    double lat_mean_us, lat_p50_us, lat_p90_us, lat_p99_us, lat_max_us;
-    double queue_p50_us, queue_p99_us; /* submit -> launch */
+    double lat_mean_us, lat_p50_us, lat_p90_us, lat_p99_us, lat_max_us; /* scheduled arrival -> done */
+    double queue_p50_us, queue_p99_us; /* scheduled arrival -> launch */
     double mean_textures_per_launch;
     uint64_t peak_inflight_bytes;
+    double gen_lag_p50_us, gen_lag_p99_us, gen_lag_max_us; /* scheduled arrival -> submit (generator lag) */
+    double submit_lat_p50_us, submit_lat_p99_us;          /* submit -> done */
 } goofy_serve_sim_result;
 
 GOOFY_HIP_API int goofy_serve_simulate(const goofy_serve_sim_config* cfg, goofy_serve_sim_result* result);
