@@ -9,6 +9,9 @@ ROOT = Path(__file__).resolve().parents[1]
 GOLDEN = ROOT / "tests" / "golden"
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
+# The reference's own test images stored under tests/golden/images/ (their
+# harness digests and quality figures are in real_images.json).
+REAL_IMAGES = sorted(k for k in json.loads((GOLDEN / "real_images.json").read_text()) if k.endswith(".png"))
 
 
 def pytest_configure(config):

@@ -7,15 +7,18 @@ and records what it outputs:
   blocks.npz   -- special and random 4x4 blocks (64 B each) with the
                   reference's DXT1 and ETC1S outputs (8 B each);
   img512.npz   -- the full DXT1 / ETC1S outputs of the 512x512 waves image;
-  real_images.json -- for three of the reference's own test images
-                  (test-data/parrot_red.png, kodim23.png, patterns.png, copied
-                  under tests/golden/images/ as data), from the reference
-                  harness (Src/main.cpp, compiled into
-                  oracle/_ref/libgoofy_harness.so): SHA-256 of the RGBA8 input
-                  as its loader returns it and of both encodings; the
-                  harness's getMsePsnr figures of the round trip through its
-                  decoders; the SHA-256 of its RGB565 quality floor and that
-                  floor's getMsePsnr figures;
+  real_images.json -- for eleven of the reference's own test images
+                  (test-data/*.png named in REAL_IMAGES, copied under
+                  tests/golden/images/ as data), from the reference harness
+                  (Src/main.cpp, compiled into oracle/_ref/libgoofy_harness.so):
+                  SHA-256 of the RGBA8 input as its loader returns it and of
+                  both encodings; the harness's getMsePsnr figures of the round
+                  trip through its decoders; the SHA-256 of its RGB565 quality
+                  floor and that floor's getMsePsnr figures.  Under
+                  "harness_table": the per-image PSNR table the harness prints
+                  (Src/main.cpp:949-979) for Goofy DXT1 / ETC1 and the floor,
+                  over every image of its long-run list (:991-1039) present in
+                  the reference tree (read from there when this script runs);
   digests.json -- SHA-256 of inputs and outputs of the 512^2, 8192^2 and the
                   pitch-padded 16384^2 stress images, and of a 16-texture
                   batch with widths that are multiples of 4 but not of 16
@@ -195,10 +198,44 @@ def write_alpha_digests(ref: Reference) -> None:
     path.write_text(json.dumps(digests, indent=1) + "\n")
 
 
+# Images stored under tests/golden/images/ (GPU tests read them there).
+REAL_IMAGES = ("parrot_red.png", "kodim23.png", "patterns.png", "baboon.png", "monarch.png", "roblox01.png",
+               "roblox05.png", "kodim03.png", "kodim20.png", "peppers.png", "lena.png")
+# The harness's long-run image list (Src/main.cpp:991-1039), in its order.
+HARNESS_LIST = (["patterns", "pbr_bricks_albedo", "pbr_ground_albedo", "pbr_stones_albedo", "pbr_stones_normal",
+                 "pbr_head_albedo", "parrot_red", "baboon", "lena", "monarch", "peppers", "sail", "tulips"] +
+                [f"kodim{i:02d}" for i in range(1, 25)] + [f"roblox{i:02d}" for i in range(1, 7)])
+REFERENCE_TEST_DATA = Path("/root/reference/test-data")
+
+
+def harness_table(ref: Reference, harness: Harness) -> dict:
+    """psnrMin / psnrRGB / psnrY of Goofy DXT1, Goofy ETC1 and the RGB565
+    floor per image, as the harness computes them, for the list's images
+    present in the reference tree; plus their plain means."""
+    rows = {}
+    for stem in HARNESS_LIST:
+        path = REFERENCE_TEST_DATA / f"{stem}.png"
+        if not path.exists():
+            continue
+        img, w, h = harness.load_png(path)
+        row = {}
+        for fmt, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
+            out = encode_image_ref(ref, fmt, img, w, h, w * 4)
+            q = harness.mse_psnr(img, harness.decompress(fmt, out, w, h), w, h)
+            row[key] = {k: q[k] for k in ("psnr_min", "psnr_rgb", "psnr_y")}
+        q = harness.mse_psnr(img, harness.rgb565_baseline(img, w, h), w, h)
+        row["floor"] = {k: q[k] for k in ("psnr_min", "psnr_rgb", "psnr_y")}
+        rows[stem] = row
+    means = {key: {k: float(np.mean([r[key][k] for r in rows.values()])) for k in ("psnr_min", "psnr_rgb", "psnr_y")}
+             for key in ("dxt1", "etc1", "floor")}
+    missing = [s for s in HARNESS_LIST if s not in rows]
+    return {"images": rows, "mean_over_present": means, "missing_from_reference_tree": missing}
+
+
 def write_real_images(ref: Reference) -> None:
     harness = Harness()
     real = {}
-    for name in ("parrot_red.png", "kodim23.png", "patterns.png"):
+    for name in REAL_IMAGES:
         img, w, h = harness.load_png(OUT / "images" / name)
         entry = {"width": w, "height": h, "input_sha256": sha(img)}
         for fmt, key in ((W.DXT1, "dxt1"), (W.ETC1, "etc1")):
@@ -212,6 +249,8 @@ def write_real_images(ref: Reference) -> None:
         entry["baseline_quality"] = harness.mse_psnr(img, base, w, h)
         real[name] = entry
         print(name, w, h, {k: round(v["psnr_rgb"], 3) for k, v in entry.items() if k.endswith("quality")})
+    real["harness_table"] = harness_table(ref, harness)
+    print("mean over present images:", real["harness_table"]["mean_over_present"])
     (OUT / "real_images.json").write_text(json.dumps(real, indent=1) + "\n")
 
 

@@ -14,6 +14,8 @@ import pytest
 
 from goofy_amd import workloads as W
 
+from conftest import REAL_IMAGES
+
 pytestmark = pytest.mark.gpu
 
 
@@ -228,7 +230,7 @@ def _check_quality(q, ref, n):
     assert q["psnr_y"] == pytest.approx(ref["psnr_y"], abs=1e-7)
 
 
-@pytest.mark.parametrize("name", ["parrot_red.png", "kodim23.png", "patterns.png"])
+@pytest.mark.parametrize("name", REAL_IMAGES)
 def test_floor_and_quality_real_images_match_harness(gpu, real_images, name):
     """GPU floor, its PSNR, and the encoder's round-trip PSNR on the
     reference's test images against the harness's own figures (fixtures)."""

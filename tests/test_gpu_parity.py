@@ -11,6 +11,8 @@ import pytest
 
 from goofy_amd import workloads as W
 
+from conftest import REAL_IMAGES
+
 pytestmark = pytest.mark.gpu
 
 
@@ -461,7 +463,7 @@ def _real():
     return GOLDEN, json.loads((GOLDEN / "real_images.json").read_text())
 
 
-@pytest.mark.parametrize("name", ["parrot_red.png", "kodim23.png", "patterns.png"])
+@pytest.mark.parametrize("name", REAL_IMAGES)
 def test_real_images_gpu(gpu, name):
     from goofy_amd import io as gio
     golden, real = _real()
@@ -478,6 +480,36 @@ def test_real_images_gpu(gpu, name):
         assert q["sse_byte"] == g[f"{key}_psnr"]["sse_byte"]
         assert q["psnr_rgb"] == pytest.approx(g[f"{key}_psnr"]["psnr_rgb"], abs=1e-9)
         assert q["psnr_y"] == pytest.approx(g[f"{key}_psnr"]["psnr_y"], abs=1e-7)
+
+
+@pytest.mark.parametrize("name", ["parrot_red.png", "roblox01.png", "patterns.png"])
+def test_containers_of_gpu_blocks_match_harness(gpu, reference, oracle, harness, tmp_path, name):
+    """DDS and KTX1 files written through goofy_container_write around blocks
+    the GPU encoded from a real image, byte for byte against the files the
+    harness's saveDds / saveKtx (Src/main.cpp:154-220) write around the
+    reference's blocks (DXT1, ETC1S) or the oracle's (DXT5, ETC2 RGBA8: the
+    reference has no alpha encoder)."""
+    from goofy_amd import io as gio
+    golden, real = _real()
+    img, w, h = gio.load_png_rgba8(golden / "images" / name)
+    src = upload(img)
+    for fmt in (W.DXT1, W.ETC1, W.DXT5, W.ETC2):
+        bb = gpu.block_bytes(fmt)
+        out = dev_buffer((w // 4) * (h // 4) * bb, fill=0xCD)
+        gpu.compress_device(fmt, out, src, w, h, w * 4)
+        got = to_host(out)
+        if fmt in (W.DXT1, W.ETC1):
+            rc, want = reference.compress(fmt, img, w, h, w * 4)
+            assert rc == 0
+        else:
+            want = oracle.compress_blocks(fmt, img, w, h, w * 4)
+        for kind in (gio.DDS, gio.KTX):
+            if kind == gio.DDS and fmt in (W.ETC1, W.ETC2):
+                continue  # the harness writes ETC formats as KTX only
+            say(f"container {name} fmt={fmt} kind={kind}")
+            path = tmp_path / f"{kind}_{fmt}"
+            harness.save(kind, fmt, path, want, w, h)
+            assert gio.container_bytes(kind, fmt, w, h, got) == path.read_bytes(), (name, fmt, kind)
 
 
 def test_device_image_beyond_4gib(gpu, oracle):

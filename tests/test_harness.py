@@ -22,9 +22,9 @@ import pytest
 from goofy_amd import io as gio
 from goofy_amd import workloads as W
 
-from conftest import GOLDEN
+from conftest import GOLDEN, REAL_IMAGES
 
-REAL = ["parrot_red.png", "kodim23.png", "patterns.png"]
+REAL = REAL_IMAGES
 
 
 def sha(a):

@@ -10,7 +10,7 @@ import pytest
 from goofy_amd import io as gio
 from goofy_amd import workloads as W
 
-from conftest import GOLDEN
+from conftest import GOLDEN, REAL_IMAGES
 
 REAL = json.loads((GOLDEN / "real_images.json").read_text())
 
@@ -19,7 +19,7 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-@pytest.mark.parametrize("name", sorted(REAL))
+@pytest.mark.parametrize("name", REAL_IMAGES)
 def test_real_images_oracle_matches_reference_digests(oracle, name):
     g = REAL[name]
     img, w, h = gio.load_png_rgba8(GOLDEN / "images" / name)
@@ -31,7 +31,7 @@ def test_real_images_oracle_matches_reference_digests(oracle, name):
         assert rc == 0 and sha(out) == g[f"{key}_sha256"]
 
 
-@pytest.mark.parametrize("name", sorted(REAL))
+@pytest.mark.parametrize("name", REAL_IMAGES)
 def test_real_images_round_trip_psnr(oracle, name):
     # harness PSNR of the oracle's decode equals the reference decoder's figures
     g = REAL[name]
