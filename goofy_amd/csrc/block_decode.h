@@ -20,13 +20,17 @@
 
 namespace goofy_dev {
 
-// 16 decoded pixels (row-major) of a BC1 block; FOUR: the colour block of
-// BC3, always in 4-colour mode.  Per channel the 4 palette bytes sit in one
-// word (byte k = entry k); a row's pixels come from one v_perm_b32 per
-// channel whose selector bytes are the pixels' 2-bit indices, and four more
-// interleave the channel rows into RGBA words.
-template <bool FOUR = false>
-__device__ __forceinline__ void decode_bc1(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
+// The decoders below hand out one pixel row at a time: emit(y, row) gets the
+// four RGBA8 words of row y (y = 0..3, in order), so a caller that consumes
+// rows as they come (a store, a comparison) keeps 4 pixels live, not 16.
+
+// BC1 block rows; FOUR: the colour block of BC3, always in 4-colour mode.
+// Per channel the 4 palette bytes sit in one word (byte k = entry k); a
+// row's pixels come from one v_perm_b32 per channel whose selector bytes are
+// the pixels' 2-bit indices, and four more interleave the channel rows into
+// RGBA words.
+template <bool FOUR, class Emit>
+__device__ __forceinline__ void bc1_rows(uint32_t w0, uint32_t w1, Emit&& emit)
 {
     const uint32_t c0 = w0 & 0xFFFFu, c1 = w0 >> 16;
     const uint32_t r0 = (c0 >> 11) & 31u, g0 = (c0 >> 5) & 63u, b0 = c0 & 31u;
@@ -55,10 +59,11 @@ __device__ __forceinline__ void decode_bc1(uint32_t w0, uint32_t w1, uint32_t (&
         const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);   // R2 G2 R3 G3
         const uint32_t ba01 = __builtin_amdgcn_perm(a, bl, 0x05010400u);  // B0 A0 B1 A1
         const uint32_t ba23 = __builtin_amdgcn_perm(a, bl, 0x07030602u);  // B2 A2 B3 A3
-        px[4 * y + 0] = __builtin_amdgcn_perm(ba01, rg01, 0x05040100u);
-        px[4 * y + 1] = __builtin_amdgcn_perm(ba01, rg01, 0x07060302u);
-        px[4 * y + 2] = __builtin_amdgcn_perm(ba23, rg23, 0x05040100u);
-        px[4 * y + 3] = __builtin_amdgcn_perm(ba23, rg23, 0x07060302u);
+        uint32_t row[4] = {__builtin_amdgcn_perm(ba01, rg01, 0x05040100u),
+                           __builtin_amdgcn_perm(ba01, rg01, 0x07060302u),
+                           __builtin_amdgcn_perm(ba23, rg23, 0x05040100u),
+                           __builtin_amdgcn_perm(ba23, rg23, 0x07060302u)};
+        emit(y, row);
     }
 }
 
@@ -112,34 +117,33 @@ __device__ __forceinline__ void etc2_th_planes(uint32_t hi, bool tmode, uint32_t
 // alpha 255.  O: red = bits 30-25 of hi, green 24 then 22-17, blue 16 then
 // 12-11 then 9-7; H: red 6-2 then 0, green = bits 31-25 of lo, blue 24-19;
 // V: red 18-13, green 12-6, blue 5-0.  6- and 7-bit values are widened by
-// replicating their top bits.
-__device__ __forceinline__ void etc2_planar(uint32_t hi, uint32_t lo, uint32_t (&px)[16])
+// replicating their top bits.  Writes the four pixels of row y.
+__device__ __forceinline__ void etc2_planar_row(uint32_t hi, uint32_t lo, int y, uint32_t (&row)[4])
 {
     const uint32_t o6[3] = {(hi >> 25) & 63u, (((hi >> 24) & 1u) << 6) | ((hi >> 17) & 63u),
                             (((hi >> 16) & 1u) << 5) | (((hi >> 11) & 3u) << 3) | ((hi >> 7) & 7u)};
     const uint32_t h6[3] = {(((hi >> 2) & 31u) << 1) | (hi & 1u), lo >> 25, (lo >> 19) & 63u};
     const uint32_t v6[3] = {(lo >> 13) & 63u, (lo >> 6) & 127u, lo & 63u};
 #pragma unroll
-    for (int y = 0; y < 4; ++y)
-#pragma unroll
-        for (int x = 0; x < 4; ++x) px[4 * y + x] = 0xFF000000u;
+    for (int x = 0; x < 4; ++x) row[x] = 0xFF000000u;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int up = c == 1 ? 1 : 2, down = c == 1 ? 6 : 4;  // 7-bit green, 6-bit red and blue
         const int32_t o = (int32_t)((o6[c] << up) | (o6[c] >> down));
         const int32_t dh = (int32_t)((h6[c] << up) | (h6[c] >> down)) - o;
         const int32_t dv = (int32_t)((v6[c] << up) | (v6[c] >> down)) - o;
+        int32_t t = 4 * o + 2 + y * dv;
+        // clamp((t >> 2), 0, 255) as clamp(t, 0, 1023) >> 2 with a logical
+        // shift: the shift-then-clamp form compiles to v_ashr_pk_u8_i32,
+        // whose upper result bits the compiler assumes zero and gfx950 does
+        // not clear
 #pragma unroll
-        for (int y = 0; y < 4; ++y) {
-            int32_t t = 4 * o + 2 + y * dv;
-#pragma unroll
-            for (int x = 0; x < 4; ++x, t += dh) px[4 * y + x] |= clamp_u8(t >> 2) << (8 * c);
-        }
+        for (int x = 0; x < 4; ++x, t += dh) row[x] |= ((uint32_t)min(max(t, 0), 1023) >> 2) << (8 * c);
     }
 }
 
-// 16 decoded pixels (row-major) of an ETC1 / ETC2 colour block; w0/w1 are
-// the block's first/second 4 bytes read as little-endian words.
+// ETC1 / ETC2 colour block rows; w0/w1 are the block's first/second 4 bytes
+// read as little-endian words.
 //
 // Per channel, the 8 clamped palette bytes of both sub-blocks sit in two
 // words (sub-block 0 in the low word, entry = selector value).  A row's four
@@ -147,8 +151,9 @@ __device__ __forceinline__ void etc2_planar(uint32_t hi, uint32_t lo, uint32_t (
 // the pixels' 2-bit selectors plus 4 for sub-block 1, and four more
 // v_perm_b32 interleave the channel rows into RGBA words (alpha 255 from
 // selector byte 0x0d).  T and H blocks use the same lookup with their four
-// paint colours in both words; planar blocks overwrite the pixels after it.
-__device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
+// paint colours in both words; planar blocks replace each row's pixels.
+template <class Emit>
+__device__ __forceinline__ void etc_rows(uint32_t w0, uint32_t w1, Emit&& emit)
 {
     const uint32_t hi = __builtin_bswap32(w0), lo = __builtin_bswap32(w1);  // big-endian words
     const bool diff = (hi >> 1) & 1u, flip = hi & 1u;
@@ -170,6 +175,17 @@ __device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (
             base[0][c] = (a4 << 4) | a4;
             base[1][c] = (b4 << 4) | b4;
         }
+    }
+    // ETC2 planar mode (blue overflow alone): rows from O, H and V, on a path
+    // of their own
+    if (__builtin_expect(overflow == 4u, 0)) {
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            uint32_t row[4];
+            etc2_planar_row(hi, lo, y, row);
+            emit(y, row);
+        }
+        return;
     }
     // plane[c][s]: byte k = clamp(base[s][c] + modifier k), k = selector value
     uint32_t plane[3][2];
@@ -209,30 +225,34 @@ __device__ __forceinline__ void decode_etc1(uint32_t w0, uint32_t w1, uint32_t (
         const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);   // R2 G2 R3 G3
         const uint32_t ba01 = __builtin_amdgcn_perm(0u, bl, 0x0d010d00u); // B0 FF B1 FF
         const uint32_t ba23 = __builtin_amdgcn_perm(0u, bl, 0x0d030d02u); // B2 FF B3 FF
-        px[4 * y + 0] = __builtin_amdgcn_perm(ba01, rg01, 0x05040100u);
-        px[4 * y + 1] = __builtin_amdgcn_perm(ba01, rg01, 0x07060302u);
-        px[4 * y + 2] = __builtin_amdgcn_perm(ba23, rg23, 0x05040100u);
-        px[4 * y + 3] = __builtin_amdgcn_perm(ba23, rg23, 0x07060302u);
+        uint32_t row[4] = {__builtin_amdgcn_perm(ba01, rg01, 0x05040100u),
+                           __builtin_amdgcn_perm(ba01, rg01, 0x07060302u),
+                           __builtin_amdgcn_perm(ba23, rg23, 0x05040100u),
+                           __builtin_amdgcn_perm(ba23, rg23, 0x07060302u)};
+        // ETC2 planar mode (blue overflow alone): the row from O, H and V
+        emit(y, row);
     }
-    if (__builtin_expect(overflow == 4u, 0)) etc2_planar(hi, lo, px);
 }
 
-// Alpha of four pixels from an 8-entry palette held as bytes of {hi:lo}:
-// v_perm_b32 with the four 3-bit codes as selector bytes.
-__device__ __forceinline__ uint32_t lookup4(uint32_t pal_hi, uint32_t pal_lo, uint32_t codes)
-{
-    return __builtin_amdgcn_perm(pal_hi, pal_lo, codes);
-}
 
-// Replace byte 3 (alpha) of the four pixels of row r with the bytes of a4.
-__device__ __forceinline__ void put_alpha(uint32_t (&px)[16], int r, uint32_t a4)
+// An alpha palette of 8 levels held as the bytes of {hi:lo}, and the block's
+// 48 code bits; row r's four alphas are one v_perm_b32 with the row's 3-bit
+// codes as selector bytes.
+struct AlphaPalette {
+    uint32_t lo, hi;
+    uint64_t bits;
+};
+
+// Replace byte 3 (alpha) of a row's four pixels with the bytes of a4.
+__device__ __forceinline__ void put_alpha(uint32_t (&row)[4], uint32_t a4)
 {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) px[4 * r + c] = __builtin_amdgcn_perm(a4, px[4 * r + c], 0x04020100u + (c << 24));
+    for (int c = 0; c < 4; ++c) row[c] = __builtin_amdgcn_perm(a4, row[c], 0x04020100u + (c << 24));
 }
 
-// BC3 alpha block (words w0, w1) into byte 3 of px.
-__device__ __forceinline__ void decode_bc3_alpha(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
+// BC3 alpha block (words w0, w1): 8 levels at sevenths when a0 > a1, else 6
+// at fifths plus 0 and 255; codes little-endian, pixel p at bits 3p.
+__device__ __forceinline__ AlphaPalette bc3_palette(uint32_t w0, uint32_t w1)
 {
     const uint32_t a0 = w0 & 0xFFu, a1 = (w0 >> 8) & 0xFFu;
     uint32_t v[8];
@@ -247,15 +267,18 @@ __device__ __forceinline__ void decode_bc3_alpha(uint32_t w0, uint32_t w1, uint3
         v[6] = 0u;
         v[7] = 255u;
     }
-    const uint32_t lo = v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24);
-    const uint32_t hi = v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24);
-    const uint64_t bits = (uint64_t)(w0 >> 16) | ((uint64_t)w1 << 16);  // pixel p at bits 3p
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t g = (uint32_t)(bits >> (12 * r));
-        const uint32_t codes = (g & 7u) | (((g >> 3) & 7u) << 8) | (((g >> 6) & 7u) << 16) | (((g >> 9) & 7u) << 24);
-        put_alpha(px, r, lookup4(hi, lo, codes));
-    }
+    AlphaPalette p;
+    p.lo = v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24);
+    p.hi = v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24);
+    p.bits = (uint64_t)(w0 >> 16) | ((uint64_t)w1 << 16);
+    return p;
+}
+
+__device__ __forceinline__ uint32_t bc3_alpha_row(const AlphaPalette& p, int r)
+{
+    const uint32_t g = (uint32_t)(p.bits >> (12 * r));
+    const uint32_t codes = (g & 7u) | (((g >> 3) & 7u) << 8) | (((g >> 6) & 7u) << 16) | (((g >> 9) & 7u) << 24);
+    return __builtin_amdgcn_perm(p.hi, p.lo, codes);
 }
 
 // ETC2 alpha (EAC) modifier magnitudes: table t holds -m0..-m3 (selectors
@@ -268,42 +291,65 @@ __device__ __forceinline__ uint32_t eac_mags(uint32_t t)
     return (uint32_t)(k[t >> 2] >> (16u * (t & 3u))) & 0xFFFFu;
 }
 
-// EAC alpha block (words w0, w1) into byte 3 of px.
-__device__ __forceinline__ void decode_eac_alpha(uint32_t w0, uint32_t w1, uint32_t (&px)[16])
+// EAC alpha block (words w0, w1): base, multiplier and table, then 48
+// big-endian selector bits, pixel (x, y) at 45 - 3(4x + y).
+__device__ __forceinline__ AlphaPalette eac_palette(uint32_t w0, uint32_t w1)
 {
     const int32_t base = (int32_t)(w0 & 0xFFu), mult = (int32_t)((w0 >> 12) & 15u);
     const uint32_t mags = eac_mags((w0 >> 8) & 15u);
-    uint32_t lo = 0, hi = 0;
+    AlphaPalette p;
+    p.lo = 0;
+    p.hi = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int32_t m = (int32_t)((mags >> (4 * k)) & 15u);
-        lo |= (uint32_t)min(max(base - m * mult, 0), 255) << (8 * k);
-        hi |= (uint32_t)min(max(base + (m - 1) * mult, 0), 255) << (8 * k);
+        p.lo |= clamp_u8(base - m * mult) << (8 * k);
+        p.hi |= clamp_u8(base + (m - 1) * mult) << (8 * k);
     }
-    // 48 selector bits, big-endian from byte 2: pixel (x, y) at 45 - 3(4x + y)
-    const uint64_t bits = ((uint64_t)__builtin_bswap32(w0 & 0xFFFF0000u) << 32 | __builtin_bswap32(w1)) &
-                          0xFFFFFFFFFFFFull;
+    p.bits = ((uint64_t)__builtin_bswap32(w0 & 0xFFFF0000u) << 32 | __builtin_bswap32(w1)) & 0xFFFFFFFFFFFFull;
+    return p;
+}
+
+__device__ __forceinline__ uint32_t eac_alpha_row(const AlphaPalette& p, int r)
+{
+    uint32_t codes = 0;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        uint32_t codes = 0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) codes |= (uint32_t)((bits >> (45 - 3 * (4 * c + r))) & 7u) << (8 * c);
-        put_alpha(px, r, lookup4(hi, lo, codes));
+    for (int c = 0; c < 4; ++c) codes |= (uint32_t)((p.bits >> (45 - 3 * (4 * c + r))) & 7u) << (8 * c);
+    return __builtin_amdgcn_perm(p.hi, p.lo, codes);
+}
+
+// Rows of a block of format FMT (b: words x, y for the 8-byte formats; alpha
+// block in x, y and colour block in z, w for DXT5 / ETC2 RGBA8).
+template <int FMT, class Emit>
+__device__ __forceinline__ void decode_block_rows(uint4 b, Emit&& emit)
+{
+    if constexpr (FMT == 0) {
+        bc1_rows<false>(b.x, b.y, emit);
+    } else if constexpr (FMT == 1) {
+        etc_rows(b.x, b.y, emit);
+    } else if constexpr (FMT == 2) {
+        const AlphaPalette p = bc3_palette(b.x, b.y);
+        bc1_rows<true>(b.z, b.w, [&](int y, uint32_t (&row)[4]) {
+            put_alpha(row, bc3_alpha_row(p, y));
+            emit(y, row);
+        });
+    } else {
+        const AlphaPalette p = eac_palette(b.x, b.y);
+        etc_rows(b.z, b.w, [&](int y, uint32_t (&row)[4]) {
+            put_alpha(row, eac_alpha_row(p, y));
+            emit(y, row);
+        });
     }
 }
 
+// The 16 pixels of a block, row-major.
 template <int FMT>
 __device__ __forceinline__ void decode_block(uint4 b, uint32_t (&px)[16])
 {
-    if constexpr (FMT == 0) decode_bc1(b.x, b.y, px);
-    else if constexpr (FMT == 1) decode_etc1(b.x, b.y, px);
-    else if constexpr (FMT == 2) {
-        decode_bc1<true>(b.z, b.w, px);
-        decode_bc3_alpha(b.x, b.y, px);
-    } else {
-        decode_etc1(b.z, b.w, px);
-        decode_eac_alpha(b.x, b.y, px);
-    }
+    decode_block_rows<FMT>(b, [&](int y, const uint32_t (&row)[4]) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) px[4 * y + c] = row[c];
+    });
 }
 
 }  // namespace goofy_dev

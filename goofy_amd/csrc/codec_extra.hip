@@ -21,7 +21,7 @@
 
 namespace goofy_impl {
 
-using goofy_dev::decode_block;
+using goofy_dev::decode_block_rows;
 
 namespace {
 
@@ -55,14 +55,11 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ bloc
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nblocks) return;
     const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
-    uint32_t px[16];
-    decode_block<FMT>(load_block_fmt<FMT>(blocks, i), px);
-    uint8_t* row = dst + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const u32x4 v = {px[4 * r], px[4 * r + 1], px[4 * r + 2], px[4 * r + 3]};
-        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(row + (uint64_t)r * pitch));
-    }
+    uint8_t* out = dst + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
+    decode_block_rows<FMT>(load_block_fmt<FMT>(blocks, i), [&](int y, const uint32_t (&row)[4]) {
+        const u32x4 v = {row[0], row[1], row[2], row[3]};
+        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(out + (uint64_t)y * pitch));
+    });
 }
 
 // Squared errors of pixel pairs, summed exactly in integers.  e[0..3] sum
@@ -170,16 +167,14 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint8_t* __restrict__ block
         for (int r = 0; r < 4; ++r)
             v[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
     };
+    // each decoded row is compared as it comes: 4 decoded pixels live at a time
     auto compare = [&](const uint4& b, const u32x4 (&v)[4]) {
-        uint32_t px[16];
-        decode_block<FMT>(b, px);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            accum_pixel(v[r].x, px[4 * r], acc);
-            accum_pixel(v[r].y, px[4 * r + 1], acc);
-            accum_pixel(v[r].z, px[4 * r + 2], acc);
-            accum_pixel(v[r].w, px[4 * r + 3], acc);
-        }
+        decode_block_rows<FMT>(b, [&](int y, const uint32_t (&row)[4]) {
+            accum_pixel(v[y].x, row[0], acc);
+            accum_pixel(v[y].y, row[1], acc);
+            accum_pixel(v[y].z, row[2], acc);
+            accum_pixel(v[y].w, row[3], acc);
+        });
     };
     // Two register sets used in turn, so that no block is copied from one
     // set to the other; a prefetch past the end reloads the last block

@@ -109,3 +109,25 @@ def test_stripe_division_magic_on_host(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "stripe magic ok" in r.stdout
+
+
+HIPCC = Path("/opt/rocm/bin/hipcc")
+
+
+@pytest.mark.skipif(not HIPCC.exists(), reason="hipcc not available")
+@pytest.mark.parametrize("source", ["kernels.hip", "codec_extra.hip", "probe.hip"])
+def test_device_code_has_no_v_ashr_pk_u8_i32(tmp_path, source):
+    """No kernel may contain v_ashr_pk_u8_i32.  The compiler makes it from an
+    arithmetic shift followed by a clamp to [0, 255] and then relies on bits
+    31:16 of its result being zero, which gfx950 does not guarantee: both the
+    generator's normal-map branch and the ETC2 planar decoder produced wrong
+    bytes on the GPU through it.  The device code is written to avoid the
+    pattern; this pins that on every build."""
+    asm = tmp_path / "k.s"
+    subprocess.run([str(HIPCC), "-O3", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature",
+                    "-Xclang", "-packed-fp32-ops", f"-I{ROOT / 'include'}", f"-I{ROOT / 'goofy_amd' / 'csrc'}",
+                    "--cuda-device-only", "-S", str(ROOT / "goofy_amd" / "csrc" / source), "-o", str(asm)],
+                   check=True, capture_output=True)
+    text = asm.read_text()
+    assert "v_lerp_u8" in text or source != "kernels.hip"  # the dump is the real kernels
+    assert "v_ashr_pk_u8_i32" not in text
