@@ -11,6 +11,9 @@
 //   * image decoders     -- decompressDXT1 / DXT5 / ETC1 / ETC2 (Src/main.cpp:561-631)
 //   * quality figures    -- getMsePsnr                   (Src/main.cpp:403-469)
 //   * quality floor      -- generateDownsamlpedRgb565Test (Src/main.cpp:501-555)
+//   * the plugin loop    -- runTestDXT1 / runTestETC1 over a CompressFunc_t
+//                           (Src/main.cpp:644-743), and the harness's own
+//                           goofy::compressDXT1 / compressETC1 to run in it
 // The harness's main() comes along in the same object; nothing calls it.
 #include "main.cpp"
 
@@ -62,4 +65,36 @@ GOOFY_HARNESS_API void goofy_harness_save(int container, int fmt, const char* pa
     static const uint32_t ktx[4] = {kKtxFormatDXT1, kKtxFormatETC1, kKtxFormatDXT5, kKtxFormatETC2};
     if (container == 0) saveDds(path, data, size, width, height, fmt == 2 ? kDdsFormatDXT5 : kDdsFormatDXT1);
     else saveKtx(path, data, size, width, height, ktx[fmt & 3]);
+}
+
+// The harness's own per-encoder loop, runTestDXT1 / runTestETC1
+// (Src/main.cpp:646-743), around any CompressFunc_t (:644): `iterations`
+// timed calls with the best kept, then the harness's decode, getMsePsnr and
+// its .tga and .dds / .ktx files under ./test-results/ of the working
+// directory.  fmt 0 = DXT1, 1 = ETC1.  time_us: the best call; mse_psnr: 14
+// doubles in the order of goofy_harness_mse_psnr; dst: the last call's blocks.
+GOOFY_HARNESS_API void goofy_harness_run_test(int fmt, const char* encoder, const char* image, CompressFunc_t func,
+                                              unsigned iterations, unsigned char* src, unsigned width,
+                                              unsigned height, unsigned stride, unsigned char* dst, size_t dst_size,
+                                              double* time_us, double* mse_psnr)
+{
+    Timer timer;
+    std::vector<unsigned char> scratch((size_t)width * height * 4);
+    const TestResult r =
+        fmt == 0 ? runTestDXT1(encoder, image, func, timer, iterations, dst, dst_size, src, width, height, stride,
+                               scratch.data())
+                 : runTestETC1(encoder, image, func, timer, iterations, dst, dst_size, src, width, height, stride,
+                               scratch.data());
+    *time_us = r.timeInMicroSeconds;
+    const MsePsnr& q = r.msePsnr;
+    const double v[14] = {q.mseR,  q.mseG,  q.mseB,  q.mseA,  q.mseMax,  q.mseRGB, q.mseY,
+                          q.psnrR, q.psnrG, q.psnrB, q.psnrA, q.psnrMin, q.psnrRGB, q.psnrY};
+    for (int i = 0; i < 14; ++i) mse_psnr[i] = v[i];
+}
+
+// goofy::compressDXT1 / compressETC1 as the harness compiles them (its
+// "simd_goofy" encoder, Src/main.cpp:925-929).
+GOOFY_HARNESS_API CompressFunc_t goofy_harness_goofy_encoder(int fmt)
+{
+    return fmt == 0 ? goofy::compressDXT1 : goofy::compressETC1;
 }

@@ -223,6 +223,12 @@ class Harness:
         lib.goofy_harness_decompress.argtypes = [ctypes.c_int, _u8p, ctypes.c_uint, ctypes.c_uint, _u8p]
         lib.goofy_harness_save.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, _u8p, ctypes.c_size_t,
                                            ctypes.c_uint, ctypes.c_uint]
+        lib.goofy_harness_run_test.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p,
+                                               ctypes.c_uint, _u8p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u8p,
+                                               ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(ctypes.c_double)]
+        lib.goofy_harness_goofy_encoder.argtypes = [ctypes.c_int]
+        lib.goofy_harness_goofy_encoder.restype = ctypes.c_void_p
         self.lib = lib
 
     def load_png(self, path) -> tuple[np.ndarray, int, int] | None:
@@ -254,6 +260,24 @@ class Harness:
         out = np.zeros(width * height * 4, dtype=np.uint8)
         self.lib.goofy_harness_decompress(fmt, _ptr(np.ascontiguousarray(blocks)), width, height, _ptr(out))
         return out
+
+    def goofy_encoder(self, fmt: int) -> int:
+        """Address of goofy::compressDXT1 (fmt 0) / compressETC1 (fmt 1) as the
+        harness compiles them: a CompressFunc_t for run_test."""
+        return int(self.lib.goofy_harness_goofy_encoder(fmt))
+
+    def run_test(self, fmt: int, encoder: str, image: str, func: int, iterations: int, img: np.ndarray,
+                 width: int, height: int, stride: int) -> tuple[np.ndarray, float, dict]:
+        """runTestDXT1 / runTestETC1 (Src/main.cpp:646-743) around the
+        CompressFunc_t at address `func`; writes the harness's files under
+        ./test-results/.  Returns (blocks, best microseconds, getMsePsnr)."""
+        src = np.ascontiguousarray(img, dtype=np.uint8)
+        dst = np.zeros((width // 4) * (height // 4) * 8, dtype=np.uint8)
+        t = ctypes.c_double(0)
+        q = (ctypes.c_double * 14)()
+        self.lib.goofy_harness_run_test(fmt, encoder.encode(), image.encode(), func, iterations, _ptr(src), width,
+                                        height, stride, _ptr(dst), dst.size, ctypes.byref(t), q)
+        return dst, t.value, dict(zip(MSE_PSNR_FIELDS, list(q)))
 
     def save(self, container: int, fmt: int, path, data: np.ndarray, width: int, height: int) -> None:
         """saveDds (container 0) / saveKtx (container 1) to `path`."""

@@ -512,6 +512,37 @@ def test_containers_of_gpu_blocks_match_harness(gpu, reference, oracle, harness,
             assert gio.container_bytes(kind, fmt, w, h, got) == path.read_bytes(), (name, fmt, kind)
 
 
+@pytest.mark.parametrize("name", ["parrot_red.png", "kodim23.png", "roblox01.png"])
+def test_harness_plugin_loop_runs_hip_entry_points(gpu, harness, tmp_path, monkeypatch, name):
+    """The reference harness's own runTestDXT1 / runTestETC1
+    (Src/main.cpp:646-743) with goofy_hip_compress_dxt1 / _etc1 in its
+    CompressFunc_t slot (:644), beside the harness's compiled
+    goofy::compressDXT1 / compressETC1 in the same loop: identical blocks,
+    getMsePsnr figures and .dds / .ktx / .tga files; the best-of-128 times of
+    both are printed."""
+    import ctypes
+
+    from conftest import GOLDEN
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "test-results").mkdir()
+    img, w, h = harness.load_png(GOLDEN / "images" / name)  # the harness's own loader
+    stem = name[:-4]
+    for fmt, key, ext, fn in ((W.DXT1, "dxt1", ".dds", gpu._lib.goofy_hip_compress_dxt1),
+                              (W.ETC1, "etc1", ".ktx", gpu._lib.goofy_hip_compress_etc1)):
+        say(f"harness loop {name} {key}")
+        addr = ctypes.cast(fn, ctypes.c_void_p).value
+        ref_blocks, ref_us, ref_q = harness.run_test(fmt, "simd_goofy", stem, harness.goofy_encoder(fmt), 128, img,
+                                                     w, h, w * 4)
+        blocks, us, q = harness.run_test(fmt, "hip_goofy", stem, addr, 128, img, w, h, w * 4)
+        assert np.array_equal(blocks, ref_blocks)
+        assert q == ref_q
+        for suffix in (f"_{key}{ext}", f"_{key}_decompressed.tga"):
+            a = (tmp_path / "test-results" / f"{stem}_hip_goofy{suffix}").read_bytes()
+            assert a == (tmp_path / "test-results" / f"{stem}_simd_goofy{suffix}").read_bytes(), suffix
+        say(f"harness loop {name} {key}: psnrRGB {q['psnr_rgb']:.3f} dB; best of 128: reference {ref_us:.0f} us "
+            f"(1 thread), HIP drop-in {us:.0f} us")
+
+
 def test_device_image_beyond_4gib(gpu, oracle):
     """32768x32768 RGBA8 (4 GiB of pixels, 67M blocks): byte offsets past 2^32.
     Stripes of block rows at the start, the middle and the very end are compared

@@ -195,3 +195,21 @@ def test_psnr_restatement_matches_get_mse_psnr(harness, oracle, real_images, nam
         assert mine["psnr_y"] == pytest.approx(ref["psnr_y"], abs=1e-9)
         n = w * h
         assert [s / n for s in mine["sse_byte"][:3]] == [ref["mse_b"], ref["mse_g"], ref["mse_r"]]
+
+
+def test_harness_plugin_loop_with_its_own_encoder(harness, tmp_path, monkeypatch, real_images):
+    """runTestDXT1 / runTestETC1 (Src/main.cpp:646-743) through the shim, with
+    the harness's own goofy::compressDXT1 / compressETC1 in the CompressFunc_t
+    slot: the blocks, getMsePsnr figures and files the harness run gives."""
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "test-results").mkdir()
+    name = "kodim23.png"
+    img, w, h = harness.load_png(GOLDEN / "images" / name)
+    for fmt, key, ext in ((W.DXT1, "dxt1", ".dds"), (W.ETC1, "etc1", ".ktx")):
+        blocks, t_us, q = harness.run_test(fmt, "simd_goofy", "kodim23", harness.goofy_encoder(fmt), 4, img, w, h,
+                                           w * 4)
+        assert sha(blocks) == real_images[name][f"{key}_sha256"]
+        assert q == real_images[name][f"{key}_quality"]
+        assert t_us > 0
+        assert (tmp_path / "test-results" / f"kodim23_simd_goofy_{key}{ext}").stat().st_size > 0
+        assert (tmp_path / "test-results" / f"kodim23_simd_goofy_{key}_decompressed.tga").stat().st_size > 0
