@@ -42,6 +42,8 @@ ERR_OUT_OF_MEMORY = -5
 ERR_NO_DEVICE = -6
 ERR_TOO_LARGE = -7
 ERR_BUSY = -8
+ERR_CANCELLED = -9
+ERR_DEADLINE = -10
 
 
 class GoofyError(RuntimeError):

@@ -156,9 +156,6 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint8_t* __restrict__ block
 {
     ErrSums acc = {};
     const uint64_t step = (uint64_t)gridDim.x * 256u;
-    // Software pipeline: the loads of a lane's next block are issued before
-    // the current block is decoded and compared, so the reads stay in flight
-    // across the arithmetic.  nblocks >= 1 (the launcher's check).
     auto fetch = [&](uint64_t i, uint4& b, u32x4 (&v)[4]) {
         const uint32_t by = (uint32_t)((i * magic) >> 40), bx = (uint32_t)i - by * bw;
         const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
@@ -167,32 +164,22 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint8_t* __restrict__ block
         for (int r = 0; r < 4; ++r)
             v[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
     };
-    // each decoded row is compared as it comes: 4 decoded pixels live at a time
-    auto compare = [&](const uint4& b, const u32x4 (&v)[4]) {
+    // One block at a time, each decoded row compared as it comes (4 decoded
+    // pixels live): at most 64 VGPRs, so registers never limit the waves per
+    // SIMD (the grid of at most 1536 workgroups is 6 per SIMD).  A software
+    // pipeline that loaded a lane's next block into a second register set
+    // before comparing the current one needed 79-82 VGPRs for the ETC formats
+    // (5-6 waves per SIMD) and was 1-3% slower.
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < nblocks; i += step) {
+        uint4 b;
+        u32x4 v[4];
+        fetch(i, b, v);
         decode_block_rows<FMT>(b, [&](int y, const uint32_t (&row)[4]) {
             accum_pixel(v[y].x, row[0], acc);
             accum_pixel(v[y].y, row[1], acc);
             accum_pixel(v[y].z, row[2], acc);
             accum_pixel(v[y].w, row[3], acc);
         });
-    };
-    // Two register sets used in turn, so that no block is copied from one
-    // set to the other; a prefetch past the end reloads the last block
-    // (in bounds, never compared).
-    const uint64_t last = nblocks - 1u;
-    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;  // i >= nblocks: the loop does not run
-    uint4 b0, b1;
-    u32x4 v0[4], v1[4];
-    fetch(i < last ? i : last, b0, v0);
-    while (i < nblocks) {
-        const uint64_t i1 = i + step;
-        fetch(i1 < last ? i1 : last, b1, v1);
-        compare(b0, v0);
-        if (i1 >= nblocks) break;
-        const uint64_t i2 = i1 + step;
-        fetch(i2 < last ? i2 : last, b0, v0);
-        compare(b1, v1);
-        i = i2;
     }
     unsigned long long e[4], ey;
     lane_totals(acc, e, ey);
