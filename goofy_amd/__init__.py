@@ -322,6 +322,21 @@ def quality_device(fmt: int, blocks, source, width: int, height: int, pitch: int
     return q.as_dict()
 
 
+_sig("goofy_hip_quality_device_noalloc", ctypes.c_int, ctypes.c_int, _vp, _vp, _uint, _uint, _sz, _vp,
+     ctypes.POINTER(Quality))
+
+
+def quality_device_noalloc(fmt: int, blocks, source, width: int, height: int, pitch: int | None = None,
+                           stream=None) -> dict:
+    """quality_device without allocating: the library's fixed per-GPU device
+    array is the work area.  Calls that overlap on one GPU race."""
+    pitch = width * 4 if pitch is None else pitch
+    q = Quality()
+    _check(_lib.goofy_hip_quality_device_noalloc(fmt, _ptr(blocks), _ptr(source), width, height, pitch,
+                                                 _stream(stream), ctypes.byref(q)))
+    return q.as_dict()
+
+
 _sig("goofy_hip_quality_images_device", ctypes.c_int, _vp, _sz, _vp, _sz, _uint, _uint, _vp,
      ctypes.POINTER(Quality))
 _sig("goofy_hip_rgb565_baseline_device", ctypes.c_int, _vp, _sz, _vp, _sz, _uint, _uint, _vp)
@@ -452,6 +467,6 @@ __all__ = [
     "compress_dxt5_fn", "compress_etc2_fn", "GoofyError", "LaunchConfig", "TextureDesc", "CompressFunc", "Batch",
     "compressDXT1", "compressETC1", "compress", "compress_device", "compress_dxt1_fn", "compress_etc1_fn",
     "synth_host", "synth_device", "last_error", "version", "device_count", "WAVES", "STRESS",
-    "decode_device", "quality_device", "compress_device_timed", "quality_images_device", "rgb565_baseline_device",
+    "decode_device", "quality_device", "quality_device_noalloc", "compress_device_timed", "quality_images_device", "rgb565_baseline_device",
     "tune_device", "compress_device_dual",
 ]

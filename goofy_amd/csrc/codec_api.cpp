@@ -63,6 +63,35 @@ static double psnr_of(double mse, double peak)
     return 10.0 * std::log10((peak * peak) / mse);
 }
 
+}  // extern "C"
+
+void goofy_impl::quality_figures(const QualitySums& h, uint64_t pixels, goofy_quality* out)
+{
+    goofy_quality q{};
+    q.pixels = pixels;
+    for (int i = 0; i < 4; ++i) q.sse_byte[i] = h.e[i];
+    q.sse_luma = (double)h.y100 / 10000.0;
+    const double n = (double)q.pixels;
+    // labels as in the reference harness: "r" = byte 2, "b" = byte 0
+    q.mse_r = (double)q.sse_byte[2] / n;
+    q.mse_g = (double)q.sse_byte[1] / n;
+    q.mse_b = (double)q.sse_byte[0] / n;
+    q.mse_a = (double)q.sse_byte[3] / n;
+    q.mse_rgb = (double)(q.sse_byte[0] + q.sse_byte[1] + q.sse_byte[2]) / n;
+    q.mse_y = q.sse_luma / n;
+    q.mse_max = std::max(std::max(q.mse_r, q.mse_g), std::max(q.mse_b, q.mse_a));
+    q.psnr_r = psnr_of(q.mse_r, 255.0);
+    q.psnr_g = psnr_of(q.mse_g, 255.0);
+    q.psnr_b = psnr_of(q.mse_b, 255.0);
+    q.psnr_a = psnr_of(q.mse_a, 255.0);
+    q.psnr_min = std::min(std::min(q.psnr_r, q.psnr_g), std::min(q.psnr_b, q.psnr_a));
+    q.psnr_rgb = psnr_of(q.mse_rgb, 768.0);
+    q.psnr_y = psnr_of(q.mse_y, 255.0);
+    *out = q;
+}
+
+extern "C" {
+
 // Device scratch of the quality sums, kept per device for reuse.  A call
 // returns its buffer only after synchronising its stream, so a buffer is never
 // shared by two calls in flight; no call frees one, because hipFree would wait
@@ -78,7 +107,7 @@ struct QualitySlot {
 
 class QualityScratchPool {
 public:
-    static constexpr size_t kBytes = kQualityPartialsOffset + (size_t)kQualityMaxGroups * sizeof(QualitySums);
+    static constexpr size_t kBytes = kQualityNoAllocBytes;
 
     // A free slot of `device`, or a new one (device scratch and pinned host
     // memory allocated).  d == nullptr on failure.
@@ -145,27 +174,7 @@ static int run_quality(uint64_t pixels, uint64_t units, hipStream_t stream, goof
     if (e != hipSuccess) return fail_hip(e, "quality readback");
     QualitySums h;
     std::memcpy(&h, slot.h, sizeof h);
-    goofy_quality q{};
-    q.pixels = pixels;
-    for (int i = 0; i < 4; ++i) q.sse_byte[i] = h.e[i];
-    q.sse_luma = (double)h.y100 / 10000.0;
-    const double n = (double)q.pixels;
-    // labels as in the reference harness: "r" = byte 2, "b" = byte 0
-    q.mse_r = (double)q.sse_byte[2] / n;
-    q.mse_g = (double)q.sse_byte[1] / n;
-    q.mse_b = (double)q.sse_byte[0] / n;
-    q.mse_a = (double)q.sse_byte[3] / n;
-    q.mse_rgb = (double)(q.sse_byte[0] + q.sse_byte[1] + q.sse_byte[2]) / n;
-    q.mse_y = q.sse_luma / n;
-    q.mse_max = std::max(std::max(q.mse_r, q.mse_g), std::max(q.mse_b, q.mse_a));
-    q.psnr_r = psnr_of(q.mse_r, 255.0);
-    q.psnr_g = psnr_of(q.mse_g, 255.0);
-    q.psnr_b = psnr_of(q.mse_b, 255.0);
-    q.psnr_a = psnr_of(q.mse_a, 255.0);
-    q.psnr_min = std::min(std::min(q.psnr_r, q.psnr_g), std::min(q.psnr_b, q.psnr_a));
-    q.psnr_rgb = psnr_of(q.mse_rgb, 768.0);
-    q.psnr_y = psnr_of(q.mse_y, 255.0);
-    *out = q;
+    quality_figures(h, pixels, out);
     return GOOFY_OK;
 }
 
@@ -190,6 +199,44 @@ int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_sou
         }
         return e;
     });
+}
+
+// As goofy_hip_quality_device with the library's fixed device array (one per
+// GPU, kQualityNoAllocBytes) as the work area, and the sums read back with a
+// plain copy into the caller's stack: no device or pinned allocation.  Calls
+// that overlap on one GPU share the array and race.
+int goofy_hip_quality_device_noalloc(int format, const void* d_blocks, const void* d_source, unsigned int width,
+                                     unsigned int height, size_t pitch, void* stream, goofy_quality* out)
+{
+    clear_error();
+    if (!out) return fail(GOOFY_ERR_INVALID, "null output");
+    int rc = check_image_device(format, d_source, d_blocks, width, height, pitch, true);
+    if (rc != GOOFY_OK) return rc;
+    if (width == 0 || height == 0) return fail(GOOFY_ERR_INVALID, "empty image");
+    const uint32_t bw = width / 4, bh = height / 4, max_rows = quality_stripe_rows(bw);
+    const uint64_t stripe_blocks = (uint64_t)bw * std::min(max_rows, bh);
+    if (quality_scratch_bytes(stripe_blocks) > kQualityNoAllocBytes)
+        return fail(GOOFY_ERR_TOO_LARGE, "quality grid larger than the fixed work area");
+    DeviceInfo di;
+    if ((rc = device_info(&di)) != GOOFY_OK) return rc;
+    void* work = nullptr;
+    hipError_t e = noalloc_scratch_address(di.device, &work);
+    if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress(quality work area)");
+    const hipStream_t s = (hipStream_t)stream;
+    for (uint32_t y0 = 0; y0 < bh && e == hipSuccess;) {  // stripes beyond one launch's range
+        const uint32_t rows = std::min(max_rows, bh - y0);
+        e = launch_sqerr(format, static_cast<const uint8_t*>(d_blocks) + (size_t)y0 * bw * block_bytes(format),
+                         static_cast<const uint8_t*>(d_source) + (size_t)y0 * 4 * pitch, pitch, width, rows * 4, work,
+                         y0 > 0, nullptr, s);
+        y0 += rows;
+    }
+    if (e != hipSuccess) return fail_hip(e, "quality kernel launch");
+    QualitySums h;
+    if ((e = hipMemcpyAsync(&h, work, sizeof h, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+        return fail_hip(e, "quality readback");
+    quality_figures(h, (uint64_t)width * height, out);
+    return GOOFY_OK;
 }
 
 // Checks shared by the two image-to-image entry points: width % 4 (one lane

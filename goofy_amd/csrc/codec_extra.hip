@@ -19,9 +19,39 @@
 #include "block_decode.h"
 #include "internal.h"
 
+#include <mutex>
+
 namespace goofy_impl {
 
 using goofy_dev::decode_block_rows;
+
+// The allocation-free quality call's work area (final sums at offset 0, one
+// partial per workgroup from kQualityPartialsOffset on), one copy per GPU for
+// the life of the process.  Host code never uses this name as an address: the
+// device address of each GPU's copy comes from hipGetSymbolAddress below.
+static __device__ __align__(256) uint8_t scratch[kQualityNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_address(int device, void** addr)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        // `device` is the calling thread's current device: the address is
+        // that GPU's copy of the array
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = p;
+    }
+    *addr = g_scratch_addr[device];
+    return hipSuccess;
+}
 
 namespace {
 
@@ -304,13 +334,13 @@ hipError_t launch_decode(int fmt, const void* blocks, void* dst, uint32_t width,
 }
 
 hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
-                        void* scratch, bool accumulate, void* host_out, hipStream_t stream)
+                        void* work, bool accumulate, void* host_out, hipStream_t stream)
 {
     const uint32_t bw = width / 4, nblocks = bw * (height / 4);
     if (nblocks == 0 || nblocks > kQualityMaxLaunchUnits) return hipErrorInvalidValue;  // 32-bit lane sums
     const uint32_t g = quality_grid(nblocks);
-    QualitySums* out = static_cast<QualitySums*>(scratch);
-    QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(scratch) + kQualityPartialsOffset);
+    QualitySums* out = static_cast<QualitySums*>(work);
+    QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(work) + kQualityPartialsOffset);
     const uint64_t magic = div_magic(bw);
     const uint8_t* b = static_cast<const uint8_t*>(blocks);
     const uint8_t* s = static_cast<const uint8_t*>(src);
@@ -326,13 +356,13 @@ hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pit
 }
 
 hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
-                               uint32_t height, void* scratch, bool accumulate, void* host_out, hipStream_t stream)
+                               uint32_t height, void* work, bool accumulate, void* host_out, hipStream_t stream)
 {
     const uint32_t qw = width / 4, n = qw * height;
     if (n == 0 || n > kQualityMaxLaunchUnits) return hipErrorInvalidValue;  // 32-bit lane sums
     const uint32_t g = quality_grid(n);
-    QualitySums* out = static_cast<QualitySums*>(scratch);
-    QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(scratch) + kQualityPartialsOffset);
+    QualitySums* out = static_cast<QualitySums*>(work);
+    QualitySums* part = reinterpret_cast<QualitySums*>(static_cast<uint8_t*>(work) + kQualityPartialsOffset);
     hipLaunchKernelGGL(k_sqerr_images, dim3(g), dim3(256), 0, stream, (const uint8_t*)a, (uint64_t)pitch_a,
                        (const uint8_t*)b, (uint64_t)pitch_b, qw, n, div_magic(qw), part);
     hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, stream, (const QualitySums*)part, g, out,

@@ -140,14 +140,24 @@ inline uint32_t quality_stripe_rows(uint64_t units_per_row)
     const uint64_t r = max_stripe_brows(units_per_row), cap = kQualityMaxLaunchUnits / (units_per_row ? units_per_row : 1);
     return (uint32_t)(r < cap ? r : cap);
 }
-// accumulate: add onto the sums already at offset 0 of scratch (later
+// accumulate: add onto the sums already at offset 0 of `work` (later
 // stripes of one image on the same stream) instead of replacing them.
 // host_out: null, or pinned host memory for a copy of the sums at offset 0
-// (one QualitySums), valid once the stream has drained.
+// (one QualitySums), valid once the stream has drained.  Every partial the
+// final kernel reads is written by the launch before it, and the first stripe
+// replaces the sums, so `work` needs no clearing between calls.
 hipError_t launch_sqerr(int fmt, const void* blocks, const void* src, size_t pitch, uint32_t width, uint32_t height,
-                        void* scratch, bool accumulate, void* host_out, hipStream_t stream);
+                        void* work, bool accumulate, void* host_out, hipStream_t stream);
 hipError_t launch_sqerr_images(const void* a, size_t pitch_a, const void* b, size_t pitch_b, uint32_t width,
-                               uint32_t height, void* scratch, bool accumulate, void* host_out, hipStream_t stream);
+                               uint32_t height, void* work, bool accumulate, void* host_out, hipStream_t stream);
+// Size of the largest quality work area (kQualityMaxGroups partials), the
+// size of the allocation-free call's fixed device array.
+constexpr size_t kQualityNoAllocBytes = kQualityPartialsOffset + (size_t)kQualityMaxGroups * sizeof(QualitySums);
+// Device address of `device`'s copy of that array (the current device of the
+// calling thread), looked up once per device and kept.
+hipError_t noalloc_scratch_address(int device, void** addr);
+// getMsePsnr's figures (Src/main.cpp:403-469) from the integer sums.
+void quality_figures(const QualitySums& sums, uint64_t pixels, goofy_quality* out);
 hipError_t launch_rgb565_baseline(const void* src, size_t src_pitch, void* dst, size_t dst_pitch, uint32_t width,
                                   uint32_t height, hipStream_t stream, hipEvent_t start = nullptr,
                                   hipEvent_t stop = nullptr);

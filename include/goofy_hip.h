@@ -206,6 +206,21 @@ typedef struct goofy_quality {
 GOOFY_HIP_API int goofy_hip_quality_device(int format, const void* d_blocks, const void* d_source, unsigned int width,
                                            unsigned int height, size_t pitch, void* stream, goofy_quality* out);
 
+/* goofy_hip_quality_device without allocating.  No counterpart in the
+ * reference: for callers that cannot allocate device or pinned memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  The kernels' partial and
+ * final sums live in a fixed 61,504-byte device array of the library, one
+ * copy per GPU held until the process ends; the sums reach `out` through a
+ * plain device-to-host copy, and the call synchronises `stream`.  It calls no
+ * hipMalloc, hipFree or pinned allocation.  Same checks, limits and results
+ * as goofy_hip_quality_device (any image that call takes fits the array:
+ * larger images run as stripes whose sums accumulate in it); every call
+ * overwrites the array, so calls that overlap on one GPU (from several
+ * threads, or on several streams) race and must be serialised by the caller. */
+GOOFY_HIP_API int goofy_hip_quality_device_noalloc(int format, const void* d_blocks, const void* d_source,
+                                                   unsigned int width, unsigned int height, size_t pitch, void* stream,
+                                                   goofy_quality* out);
+
 /* getMsePsnr (Src/main.cpp:403-469) between two device RGBA8 images of the
  * same size, e.g. a source and its quality floor below: same figures and
  * labels as goofy_hip_quality_device.  width % 4 == 0; rows 16-byte aligned.
