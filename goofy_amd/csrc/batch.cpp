@@ -3,10 +3,12 @@
 // A plan turns n device-resident textures of uneven sizes into one flattened
 // work-list: texture t owns wave tiles [tile_begin(t), tile_begin(t+1)), a
 // prefix sum over ceil(blocks(t) / tile).  Tiles never straddle textures, so
-// a wave's loads stay contiguous and its format is uniform.  One persistent
-// kernel walks the list (kernels.hip, k_encode_batch); the launch is captured
-// once into a hipGraph and replayed.  The launch shape (blocks per lane,
-// workgroup size, persistent waves per CU, streaming hints) comes from an
+// a wave's loads stay contiguous and its format is uniform.  One kernel launch
+// walks the whole list (kernels.hip, k_encode_batch), one wave per tile or a
+// persistent grid of waves that each take a contiguous run of tiles; the
+// launch is captured once into a hipGraph and replayed.  The launch shape
+// (blocks per lane, workgroup size, persistent waves per CU, streaming hints)
+// comes from an
 // autotuner that times the candidates on the plan itself and caches the
 // winner per shape class for the process (and, with GOOFY_TUNE_CACHE=<file>,
 // across processes).

@@ -84,6 +84,8 @@ __device__ __forceinline__ uint32_t q5(uint32_t c) { return (c - (c != 0u ? 1u :
 struct BlockStats {
     uint32_t mn, mx;      // packed RGBA bounding-box corners
     uint32_t range, mid, qt;
+    uint32_t mnrb, mxrb;  // R, B of the corners in 16-bit lanes
+    uint32_t mng, mxg;    // G of the corners
 };
 
 __device__ __forceinline__ BlockStats block_stats(const uint32_t (&p)[16])
@@ -108,6 +110,10 @@ __device__ __forceinline__ BlockStats block_stats(const uint32_t (&p)[16])
     mng = pk_min16(mng, mng >> 16) & 0xFFu;
     mxg = pk_max16(mxg, mxg >> 16) & 0xFFu;
     BlockStats s;
+    s.mnrb = mnrb;
+    s.mxrb = mxrb;
+    s.mng = mng;
+    s.mxg = mxg;
     s.mn = mnrb | (mng << 8);
     s.mx = mxrb | (mxg << 8);
     const uint32_t ymn = luma_b0(s.mn);
@@ -130,6 +136,26 @@ __device__ __forceinline__ void compare4(uint32_t y4, uint32_t nmid4, uint32_t n
 {
     *gez = avg4(nmid4, y4);
     *near = avg4(nlo4, y4) & avg4(~y4, hi4);
+}
+
+// q5 of the two 16-bit lanes of x: subsat(c, 1) >> 3 per lane.
+__device__ __forceinline__ uint32_t q5x2(uint32_t x)
+{
+    const u16x2 one = {1, 1}, three = {3, 3};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, x), one) >> three);
+}
+
+// The DXT1 endpoint word c0 | c1 << 16: c0 = RGB565 of the box maximum with
+// the green LSB forced to 1 (so c0 > c1, 4-colour mode), c1 = RGB565 of the
+// minimum.  All six channels go through q5 in 16-bit lanes: [R, B] of the
+// maximum, [R, B] of the minimum and [G max, G min]; two v_perm_b32 pair the
+// reds and the blues of both endpoints.
+__device__ __forceinline__ uint32_t dxt1_endpoints(const BlockStats& s)
+{
+    const uint32_t x = q5x2(s.mxrb), n = q5x2(s.mnrb), g = q5x2(s.mxg | (s.mng << 16));
+    const uint32_t r = perm(n, x, 0x0c040c00u);  // [R max, R min] in the two lanes
+    const uint32_t b = perm(n, x, 0x0c060c02u);  // [B max, B min]
+    return (r << 11) | (g << 6) | b | 0x20u;
 }
 
 // DXT1 (BC1) block: returns {c0 | c1<<16, indices} as two little-endian words.
@@ -155,10 +181,34 @@ __device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16])
         t = t | (t >> 12);
         idx |= (t & 0xFFu) << (8 * r);
     }
-    // RGB565 endpoints; green LSB of c0 set so c0 > c1 (4-colour mode)
-    const uint32_t c0 = 0x20u | (q5(s.mx & 0xFFu) << 11) | (q5((s.mx >> 8) & 0xFFu) << 6) | q5((s.mx >> 16) & 0xFFu);
-    const uint32_t c1 = (q5(s.mn & 0xFFu) << 11) | (q5((s.mn >> 8) & 0xFFu) << 6) | q5((s.mn >> 16) & 0xFFu);
-    return make_uint2(c0 | (c1 << 16), idx);
+    return make_uint2(dxt1_endpoints(s), idx);
+}
+
+// ETC1 table index and base-colour bytes of the block, from its average
+// colour: each channel of avgc moved by the luma correction (up: + corr,
+// else - corr) with saturation to 0..255 (goofy_tc.h:1431-1449), then cut to
+// 5 bits in place, q5(b) << 3 == subsat(b, 1) & 0xf8.  R and B run in the two
+// 16-bit lanes of one word, G in another: signed 16-bit adds, clamps with
+// v_pk_max_i16 / v_pk_min_i16 and a saturating v_pk_sub_u16.
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t etc1_base_bytes(uint32_t avgc, uint32_t mid)
+{
+    const uint32_t ya = luma_b0(avgc);
+    const bool up = ya <= mid;
+    uint32_t corr = up ? mid - ya : ya - mid;
+    corr = corr < 127u ? corr : 127u;
+    const uint32_t sd = up ? corr : 0u - corr;   // low 16 bits: the signed correction
+    const uint32_t sd2 = perm(0u, sd, 0x01000100u);  // in both lanes
+    const i16x2 zero = {0, 0}, top = {255, 255};
+    const u16x2 one = {1, 1};
+    auto base = [&](uint32_t lanes) {  // lanes: two channels in 16-bit lanes, 0..255 each
+        const i16x2 v = __builtin_bit_cast(i16x2, lanes) + __builtin_bit_cast(i16x2, sd2);
+        const i16x2 c = __builtin_elementwise_min(__builtin_elementwise_max(v, zero), top);
+        return __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, c), one)) &
+               0x00F800F8u;
+    };
+    return base(avgc & 0x00FF00FFu) | ((base((avgc >> 8) & 0x00FF00FFu) & 0xFFu) << 8);
 }
 
 // ETC1 codeword-table index for the brightness range (breakpoints of
@@ -204,19 +254,7 @@ __device__ __forceinline__ uint2 encode_etc1(const uint32_t (&p)[16])
     for (int x = 0; x < 4; ++x) col[x] = avg4(avg4(p[x], p[4 + x]), avg4(p[8 + x], p[12 + x]));
     const uint32_t avgc = avg4(avg4(col[0], col[1]), avg4(col[2], col[3]));
     // Shift its brightness onto mid (goofy_tc.h:1431-1449).
-    const uint32_t ya = luma_b0(avgc);
-    const bool up = ya <= s.mid;
-    uint32_t corr = up ? s.mid - ya : ya - s.mid;
-    corr = corr < 127u ? corr : 127u;
-    uint32_t w0 = 0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const uint32_t a = (avgc >> (8 * c)) & 0xFFu;
-        uint32_t b = up ? a + corr : (a > corr ? a - corr : 0u);
-        b = b < 255u ? b : 255u;
-        // q5(b) << 3 == subsat(b,1) & 0xf8
-        w0 |= ((b - (b != 0u ? 1u : 0u)) & 0xF8u) << (8 * c);
-    }
+    uint32_t w0 = etc1_base_bytes(avgc, s.mid);
     const uint32_t cw = etc1_table(s.range);
     w0 |= ((cw << 5) | (cw << 2) | 3u) << 24;
     return make_uint2(w0, w1);
@@ -264,11 +302,7 @@ __device__ __forceinline__ uint2 pack_dxt1(const BlockClass& c)
         t = t | (t >> 12);
         idx |= (t & 0xFFu) << (8 * r);
     }
-    // RGB565 endpoints; green LSB of c0 set so c0 > c1 (4-colour mode)
-    const BlockStats& s = c.s;
-    const uint32_t c0 = 0x20u | (q5(s.mx & 0xFFu) << 11) | (q5((s.mx >> 8) & 0xFFu) << 6) | q5((s.mx >> 16) & 0xFFu);
-    const uint32_t c1 = (q5(s.mn & 0xFFu) << 11) | (q5((s.mn >> 8) & 0xFFu) << 6) | q5((s.mn >> 16) & 0xFFu);
-    return make_uint2(c0 | (c1 << 16), idx);
+    return make_uint2(dxt1_endpoints(c.s), idx);
 }
 
 // ETC1S words from a classified block and its pixels (as encode_etc1).
@@ -295,19 +329,7 @@ __device__ __forceinline__ uint2 pack_etc1(const uint32_t (&p)[16], const BlockC
     for (int x = 0; x < 4; ++x) col[x] = avg4(avg4(p[x], p[4 + x]), avg4(p[8 + x], p[12 + x]));
     const uint32_t avgc = avg4(avg4(col[0], col[1]), avg4(col[2], col[3]));
     // Shift its brightness onto mid (goofy_tc.h:1431-1449).
-    const uint32_t ya = luma_b0(avgc);
-    const bool up = ya <= c.s.mid;
-    uint32_t corr = up ? c.s.mid - ya : ya - c.s.mid;
-    corr = corr < 127u ? corr : 127u;
-    uint32_t w0 = 0;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        const uint32_t a = (avgc >> (8 * ch)) & 0xFFu;
-        uint32_t b = up ? a + corr : (a > corr ? a - corr : 0u);
-        b = b < 255u ? b : 255u;
-        // q5(b) << 3 == subsat(b,1) & 0xf8
-        w0 |= ((b - (b != 0u ? 1u : 0u)) & 0xF8u) << (8 * ch);
-    }
+    uint32_t w0 = etc1_base_bytes(avgc, c.s.mid);
     const uint32_t cw = etc1_table(c.s.range);
     w0 |= ((cw << 5) | (cw << 2) | 3u) << 24;
     return make_uint2(w0, w1);

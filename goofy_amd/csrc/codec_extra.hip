@@ -23,6 +23,7 @@
 
 namespace goofy_impl {
 
+using goofy_dev::decode_block;
 using goofy_dev::decode_block_rows;
 
 // The allocation-free quality call's work area (final sums at offset 0, one
@@ -85,11 +86,16 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ bloc
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nblocks) return;
     const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
+    // the four rows are stored together after the block is decoded: storing
+    // each row as it comes made the decoders 2-7% slower
+    uint32_t px[16];
+    decode_block<FMT>(load_block_fmt<FMT>(blocks, i), px);
     uint8_t* out = dst + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
-    decode_block_rows<FMT>(load_block_fmt<FMT>(blocks, i), [&](int y, const uint32_t (&row)[4]) {
-        const u32x4 v = {row[0], row[1], row[2], row[3]};
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        const u32x4 v = {px[4 * y], px[4 * y + 1], px[4 * y + 2], px[4 * y + 3]};
         __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(out + (uint64_t)y * pitch));
-    });
+    }
 }
 
 // Squared errors of pixel pairs, summed exactly in integers.  e[0..3] sum

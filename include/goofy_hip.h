@@ -257,7 +257,9 @@ GOOFY_HIP_API int goofy_hip_traffic_probe(void* d_out, const void* d_input, unsi
 /* ---- batched encode: many textures, one launch -----------------------------
  * A plan flattens the block work-lists of n device-resident textures (each
  * its own size, pitch and format) into one wave-tile list and encodes all of
- * them with one persistent kernel launch, captured once into a hipGraph. */
+ * them with one kernel launch, captured once into a hipGraph.  The launch
+ * runs one wave per tile, or a persistent grid (waves_per_cu > 0) in which
+ * each wave takes a contiguous run of tiles; the autotuner times both. */
 typedef struct goofy_texture_desc {
     const void* src;     /* device RGBA8, 16-byte aligned */
     void* dst;           /* device, (width/4)*(height/4) blocks, aligned to the block size */
