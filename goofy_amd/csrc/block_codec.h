@@ -396,31 +396,30 @@ __device__ __forceinline__ void inv_midpoints(const uint32_t (&nlv)[8], uint32_t
     for (int k = 1; k < 8; ++k) nthr4[k - 1] = __builtin_amdgcn_lerp(r[k - 1], r[k], 0u);
 }
 
-// Rank bits of four alphas.  m[k] has bit 7 of byte c set when alpha c
-// reaches thr_{k+1}; the thresholds ascend, so the seven flags of a byte are
-// a thermometer code and its count t = sum of flags has the bits
-// t2 = flag 4, t1 = flag 2 ^ flag 4 ^ flag 6, t0 = xor of all seven.
-struct Rank {
-    uint32_t t0, t1, t2;  // bit 7 of byte c: bit j of alpha c's rank
-};
-
-__device__ __forceinline__ Rank rank_bits(uint32_t w, const uint32_t (&nthr4)[7])
+// Threshold flags of four alphas: m[k] has bit 7 of byte c set when alpha c
+// reaches thr_{k+1} (f_{k+1} below); the other bits are don't-care.  The
+// thresholds ascend, so the flags of a byte are a thermometer code f_1 >= f_2
+// >= ... >= f_7 and its rank t (0..7) is their count.
+__device__ __forceinline__ void rank_flags(uint32_t w, const uint32_t (&nthr4)[7], uint32_t (&m)[7])
 {
-    uint32_t m[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) m[k] = avg4(nthr4[k], w);  // avg(~t, v) >= 128 <=> v >= t
-    Rank r;
-    r.t2 = m[3];
-    r.t1 = xor3(m[1], m[3], m[5]);
-    r.t0 = xor3(xor3(m[0], m[2], m[4]), m[6], r.t1);
-    return r;
 }
 
-// Index bits (bit 7 of each byte) -> 3-bit index in bits 5..7 of each byte.
-__device__ __forceinline__ uint32_t gather3(uint32_t i0, uint32_t i1, uint32_t i2)
+// Bits of m where `sel` is set, bits of o elsewhere: one v_bitop3_b32
+// (truth table indexed by sel*4 + m*2 + o).  Written as the builtin so that
+// the compiler keeps the single select instead of re-deriving masks from
+// the known-zero bits of shifted operands.
+__device__ __forceinline__ uint32_t bsel(uint32_t sel, uint32_t m, uint32_t o)
 {
-    const uint32_t x = (i2 & 0x80808080u) | ((i1 >> 1) & 0x40404040u);
-    return (x & 0xC0C0C0C0u) | ((i0 >> 2) & 0x20202020u);
+    return __builtin_amdgcn_bitop3_b32(sel, m, o, 0xCAu);
+}
+
+// Three selector bits at bit 7 of each byte -> the 3-bit selector in bits
+// 5..7 of the byte (bit 2 from s2); bits 0..4 are left as don't-care.
+__device__ __forceinline__ uint32_t sel3(uint32_t s2, uint32_t s1, uint32_t s0)
+{
+    return bsel(0x80808080u, s2, bsel(0x40404040u, s1 >> 1, s0 >> 2));
 }
 
 // BC3 alpha block as two little-endian words, from the block's alpha rows.
@@ -435,25 +434,31 @@ __device__ __forceinline__ uint2 bc3_alpha(const AlphaRows& a)
     nlv[7] = 255u - a.mx;
     uint32_t nthr4[7];
     inv_midpoints(nlv, nthr4);
-    uint32_t v[4];
+    uint32_t g[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const Rank t = rank_bits(a.w[r], nthr4);
-        // rank -> index: 0 -> 1 (a1), 7 -> 0 (a0), k -> 8 - k in between;
-        // 8 - t = -t mod 8 has bits t0, t1^t0, t2^(t1|t0), and t0 flips
-        // where t0 == t1 == t2 (t = 0 or 7)
-        const uint32_t i0 = t.t0 ^ ~((t.t0 ^ t.t1) | (t.t1 ^ t.t2));
-        const uint32_t i1 = t.t1 ^ t.t0;
-        const uint32_t i2 = t.t2 ^ (t.t1 | t.t0);
-        // index of column c in bits 8c+5..8c+7 -> 12 bits, column c at 3c
-        const uint32_t z = gather3(i0, i1, i2);
-        const uint32_t y = z | (z >> 5);
-        v[r] = ((y >> 5) & 0x3Fu) | (((y >> 21) & 0x3Fu) << 6);
+        uint32_t m[7];
+        rank_flags(a.w[r], nthr4, m);
+        // rank t -> selector: 0 -> 1 (a1), 7 -> 0 (a0), 8 - t in between:
+        //   t    0 1 2 3 4 5 6 7
+        //   s2   0 1 1 1 1 0 0 0   = f1 & ~f5
+        //   s1   0 1 1 0 0 1 1 0   = f1 ^ f3 ^ f5 ^ f7
+        //   s0   1 1 0 1 0 1 0 0   = (f1 ^ ... ^ f6) | ~f1
+        const uint32_t x = xor3(m[0], m[2], m[4]);
+        const uint32_t s2 = m[0] & ~m[4], s1 = x ^ m[6];
+        const uint32_t s0 = (x ^ xor3(m[1], m[3], m[5])) | ~m[0];
+        const uint32_t z = sel3(s2, s1, s0);
+        // columns 0, 1 -> bits 5..10 and 2, 3 -> bits 21..26, then all four
+        // -> bits 5..16, column c at 5 + 3c (other bits don't-care)
+        const uint32_t y = bsel(0x00E000E0u, z, z >> 5);
+        g[r] = bsel(0x000007E0u, y, y >> 10);
     }
-    // row r at bits 12r..12r+11 of the 48 index bits (after a0, a1)
-    const uint32_t lo = v[0] | (v[1] << 12) | (v[2] << 24);
-    const uint32_t hi = (v[2] >> 8) | (v[3] << 4);
-    return make_uint2(a.mx | (a.mn << 8) | (lo << 16), (lo >> 16) | (hi << 16));
+    // row r at bits 12r..12r+11 of the 48 index bits after a0, a1: word 0
+    // takes row 0 and the low 4 bits of row 1, word 1 the rest
+    const uint32_t head = a.mx | (a.mn << 8);
+    const uint32_t w0 = bsel(0xFFFFu, head, bsel(0x0FFF0000u, g[0] << 11, g[1] << 23));
+    const uint32_t w1 = bsel(0xFFu, g[1] >> 9, bsel(0xFFF00u, g[2] << 3, g[3] << 15));
+    return make_uint2(w0, w1);
 }
 
 __device__ __forceinline__ uint2 encode_bc3_alpha(const uint32_t (&p)[16]) { return bc3_alpha(alpha_rows(p)); }
@@ -479,14 +484,18 @@ __device__ __forceinline__ uint2 eac_alpha(const AlphaRows& a)
     uint32_t ge = 0, go = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const Rank t = rank_bits(a.w[r], nthr4);
-        // rank -> index: 0..3 -> 3..0 (negative modifiers), 4..7 unchanged
-        const uint32_t z = gather3(~(t.t0 ^ t.t2), ~(t.t1 ^ t.t2), t.t2);
-        // column 0 / 2 index at bits 5..7 / 21..23 -> 9-3r / 25-3r;
+        uint32_t m[7];
+        rank_flags(a.w[r], nthr4, m);
+        // rank t -> selector: 0..3 -> 3..0 (negative modifiers), 4..7
+        // unchanged: s2 = f4, s1 = ~(f2 ^ f6), s0 = ~(f1 ^ f2 ^ f3 ^ f5 ^ f6 ^ f7)
+        const uint32_t s1 = ~(m[1] ^ m[5]);
+        const uint32_t s0 = ~(xor3(m[0], m[1], m[2]) ^ xor3(m[4], m[5], m[6]));
+        const uint32_t z = sel3(m[3], s1, s0);
+        // column 0 / 2 selector at bits 5..7 / 21..23 -> 9-3r / 25-3r;
         // column 1 / 3 at bits 13..15 / 29..31 -> 9-3r / 25-3r
-        const uint32_t e = z & 0x00E000E0u, o = z & 0xE000E000u;
-        ge |= r < 2 ? e << (4 - 3 * r) : e >> (3 * r - 4);
-        go |= o >> (4 + 3 * r);
+        const uint32_t dst = 0x0E000E00u >> (3 * r);
+        ge = bsel(dst, r < 2 ? z << (4 - 3 * r) : z >> (3 * r - 4), ge);
+        go = bsel(dst, z >> (4 + 3 * r), go);
     }
     // 48 bits, big-endian: column 0 (x = 0) first, 12 bits per column
     const uint32_t hi = ((ge & 0xFFFu) << 12) | (go & 0xFFFu);  // columns 0, 1

@@ -85,6 +85,53 @@ def test_alpha_launch_configs_vs_oracle(gpu, oracle, bpl, wg, nt, order):
                               oracle.compress_blocks(fmt, img, w, h, pitch))
 
 
+@pytest.mark.parametrize("bpl", [1, 2, 4])
+@pytest.mark.parametrize("w", [400, 200])
+def test_narrow_textures_all_formats_vs_oracle(gpu, oracle, bpl, w):
+    # 100 and 50 blocks a row: below 64 * blocks-per-lane a tile spans more
+    # than two block rows and every lane forms its own 64-bit address; at 100
+    # blocks and one block a lane, tiles straddle two rows on the wave-uniform
+    # base path
+    h, pitch = 84, w * 4 + 32
+    img = gpu.synth_host(W.WAVES_ALPHA, 77 + w, 0, 25, w, h, pitch, fill=0x3C)
+    src = upload(img)
+    cfg = dict(blocks_per_lane=bpl, wg_waves=4, waves_per_cu=0, nontemporal=1, tile_order=1)
+    for fmt in (W.DXT1, W.ETC1, W.DXT5, W.ETC2):
+        say(f"narrow {w}x{h} config {cfg} fmt={fmt}")
+        assert np.array_equal(encode_dev(gpu, fmt, src, w, h, pitch, cfg),
+                              oracle.compress_blocks(fmt, img, w, h, pitch))
+
+
+@pytest.mark.parametrize("pitch", [(1 << 29) - 16, 1 << 29])
+def test_pitch_either_side_of_the_32_bit_offset_limit(gpu, oracle, pitch):
+    # A pixel's byte offset from its tile's first block row reaches
+    # 7 * pitch + 16 * bw; below pitch = 2^29 the kernels address a tile from
+    # one wave-uniform base with 32-bit offsets (here up to 3.7 GB), from there
+    # on with per-lane 64-bit addresses.  68 blocks a row: tiles straddle the
+    # two block rows.  About 3.7 GB of device memory.
+    w, h, tight = 272, 8, 272 * 4
+    img = gpu.synth_host(W.WAVES_ALPHA, 31, 0, 25, w, h, tight, fill=0)
+    big = dev_buffer(pitch * (h - 1) + tight)
+    gpu.synth_device(W.WAVES_ALPHA, 31, 0, 25, big, w, h, pitch)
+    try:
+        for bpl in (1, 2):
+            cfg = dict(blocks_per_lane=bpl, wg_waves=2, waves_per_cu=0, nontemporal=1, tile_order=0)
+            for fmt in (W.DXT1, W.ETC1, W.DXT5, W.ETC2):
+                say(f"pitch {pitch:#x} bpl={bpl} fmt={fmt}")
+                assert np.array_equal(encode_dev(gpu, fmt, big, w, h, pitch, cfg),
+                                      oracle.compress_blocks(fmt, img, w, h, tight))
+        for fn, fa, fb in ((gpu.compress_device_dual, W.DXT1, W.ETC1),
+                           (gpu.compress_device_dual_alpha, W.DXT5, W.ETC2)):
+            na, nb = (w // 4) * (h // 4) * gpu.block_bytes(fa), (w // 4) * (h // 4) * gpu.block_bytes(fb)
+            a, b = dev_buffer(na, fill=0xCD), dev_buffer(nb, fill=0xCD)
+            say(f"pitch {pitch:#x} {fn.__name__}")
+            fn(a, b, big, w, h, pitch)
+            assert np.array_equal(to_host(a)[:na], oracle.compress_blocks(fa, img, w, h, tight))
+            assert np.array_equal(to_host(b)[:nb], oracle.compress_blocks(fb, img, w, h, tight))
+    finally:
+        big.free()
+
+
 def test_alpha_striped(gpu, oracle):
     """2^19-block rows: the image runs as stripes of 3 + 1 block rows."""
     from blockgen import block_image
