@@ -88,27 +88,10 @@ struct BlockStats {
     uint32_t mng, mxg;    // G of the corners
 };
 
-__device__ __forceinline__ BlockStats block_stats(const uint32_t (&p)[16])
+// The rest of the statistics from the bounding-box corners: R, B in the
+// 16-bit lanes of mnrb / mxrb, G in mng / mxg.
+__device__ __forceinline__ BlockStats corner_stats(uint32_t mnrb, uint32_t mxrb, uint32_t mng, uint32_t mxg)
 {
-    // Bounding box: R,B in the two 16-bit halves of p & 0x00ff00ff; G of two
-    // pixels packed into one word with v_perm ([G_a 0 G_b 0]).
-    uint32_t mnrb = p[0] & 0x00FF00FFu, mxrb = mnrb;
-#pragma unroll
-    for (int i = 1; i < 16; ++i) {
-        const uint32_t rb = p[i] & 0x00FF00FFu;
-        mnrb = pk_min16(mnrb, rb);
-        mxrb = pk_max16(mxrb, rb);
-    }
-    uint32_t g = perm(p[1], p[0], 0x0c050c01u);
-    uint32_t mng = g, mxg = g;
-#pragma unroll
-    for (int i = 2; i < 16; i += 2) {
-        g = perm(p[i + 1], p[i], 0x0c050c01u);
-        mng = pk_min16(mng, g);
-        mxg = pk_max16(mxg, g);
-    }
-    mng = pk_min16(mng, mng >> 16) & 0xFFu;
-    mxg = pk_max16(mxg, mxg >> 16) & 0xFFu;
     BlockStats s;
     s.mnrb = mnrb;
     s.mxrb = mxrb;
@@ -126,6 +109,57 @@ __device__ __forceinline__ BlockStats block_stats(const uint32_t (&p)[16])
     const uint32_t eighth = (quarter + 1u) >> 1;
     s.qt = quarter + eighth;  // <= 96
     return s;
+}
+
+// R, B bounding box in the two 16-bit halves of p & 0x00ff00ff.
+__device__ __forceinline__ void rb_box(const uint32_t (&p)[16], uint32_t& mnrb, uint32_t& mxrb)
+{
+    mnrb = p[0] & 0x00FF00FFu;
+    mxrb = mnrb;
+#pragma unroll
+    for (int i = 1; i < 16; ++i) {
+        const uint32_t rb = p[i] & 0x00FF00FFu;
+        mnrb = pk_min16(mnrb, rb);
+        mxrb = pk_max16(mxrb, rb);
+    }
+}
+
+__device__ __forceinline__ BlockStats block_stats(const uint32_t (&p)[16])
+{
+    uint32_t mnrb, mxrb;
+    rb_box(p, mnrb, mxrb);
+    // G of two pixels packed into one word with v_perm ([G_a 0 G_b 0])
+    uint32_t g = perm(p[1], p[0], 0x0c050c01u);
+    uint32_t mng = g, mxg = g;
+#pragma unroll
+    for (int i = 2; i < 16; i += 2) {
+        g = perm(p[i + 1], p[i], 0x0c050c01u);
+        mng = pk_min16(mng, g);
+        mxg = pk_max16(mxg, g);
+    }
+    mng = pk_min16(mng, mng >> 16) & 0xFFu;
+    mxg = pk_max16(mxg, mxg >> 16) & 0xFFu;
+    return corner_stats(mnrb, mxrb, mng, mxg);
+}
+
+// The same for a block whose alpha range is needed too: v_pk_min_u16 /
+// v_pk_max_u16 over the raw pixel words order each 16-bit lane by its high
+// byte first, so the lanes' high bytes come out as the G range (byte 1) and
+// the alpha range (byte 3) together, in the 30 instructions that the G
+// pairs and a separate alpha pass would take 44.
+__device__ __forceinline__ BlockStats block_stats_alpha(const uint32_t (&p)[16], uint32_t& amn, uint32_t& amx)
+{
+    uint32_t mnrb, mxrb;
+    rb_box(p, mnrb, mxrb);
+    uint32_t lo = p[0], hi = p[0];
+#pragma unroll
+    for (int i = 1; i < 16; ++i) {
+        lo = pk_min16(lo, p[i]);
+        hi = pk_max16(hi, p[i]);
+    }
+    amn = lo >> 24;
+    amx = hi >> 24;
+    return corner_stats(mnrb, mxrb, (lo >> 8) & 0xFFu, (hi >> 8) & 0xFFu);
 }
 
 // Per-byte comparison masks for four packed lumas y4 against the block's mid
@@ -159,9 +193,8 @@ __device__ __forceinline__ uint32_t dxt1_endpoints(const BlockStats& s)
 }
 
 // DXT1 (BC1) block: returns {c0 | c1<<16, indices} as two little-endian words.
-__device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16])
+__device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16], const BlockStats& s)
 {
-    const BlockStats s = block_stats(p);
     const uint32_t lo = s.mid + 1u > s.qt ? s.mid + 1u - s.qt : 0u;          // y >= lo
     const uint32_t hi = s.mid + s.qt - 1u < 255u ? s.mid + s.qt - 1u : 255u;  // y <= hi
     const uint32_t nmid4 = rep4(~s.mid);
@@ -183,6 +216,8 @@ __device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16])
     }
     return make_uint2(dxt1_endpoints(s), idx);
 }
+
+__device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16]) { return encode_dxt1(p, block_stats(p)); }
 
 // ETC1 table index and base-colour bytes of the block, from its average
 // colour: each channel of avgc moved by the luma correction (up: + corr,
@@ -221,9 +256,8 @@ __device__ __forceinline__ uint32_t etc1_table(uint32_t range)
 
 // ETC1S block: differential mode, zero deltas, one table for both halves,
 // flip = 1; returns the 8 bytes as two little-endian words.
-__device__ __forceinline__ uint2 encode_etc1(const uint32_t (&p)[16])
+__device__ __forceinline__ uint2 encode_etc1(const uint32_t (&p)[16], const BlockStats& s)
 {
-    const BlockStats s = block_stats(p);
     const uint32_t lo = s.mid + 1u > s.qt ? s.mid + 1u - s.qt : 0u;
     const uint32_t hi = s.mid + s.qt - 1u < 255u ? s.mid + s.qt - 1u : 255u;
     const uint32_t nmid4 = rep4(~s.mid);
@@ -260,6 +294,8 @@ __device__ __forceinline__ uint2 encode_etc1(const uint32_t (&p)[16])
     return make_uint2(w0, w1);
 }
 
+__device__ __forceinline__ uint2 encode_etc1(const uint32_t (&p)[16]) { return encode_etc1(p, block_stats(p)); }
+
 // ---- both formats from one classification (the dual-format kernel) --------
 // The same arithmetic as encode_dxt1 / encode_etc1 above, split into the part
 // both formats share -- statistics and, per row r, the per-byte masks of
@@ -272,10 +308,10 @@ struct BlockClass {
     uint32_t gez[4], near[4];
 };
 
-__device__ __forceinline__ BlockClass classify(const uint32_t (&p)[16])
+__device__ __forceinline__ BlockClass classify(const uint32_t (&p)[16], const BlockStats& s)
 {
     BlockClass c;
-    c.s = block_stats(p);
+    c.s = s;
     const uint32_t lo = c.s.mid + 1u > c.s.qt ? c.s.mid + 1u - c.s.qt : 0u;          // y >= lo
     const uint32_t hi = c.s.mid + c.s.qt - 1u < 255u ? c.s.mid + c.s.qt - 1u : 255u;  // y <= hi
     const uint32_t nmid4 = rep4(~c.s.mid);
@@ -288,6 +324,8 @@ __device__ __forceinline__ BlockClass classify(const uint32_t (&p)[16])
     }
     return c;
 }
+
+__device__ __forceinline__ BlockClass classify(const uint32_t (&p)[16]) { return classify(p, block_stats(p)); }
 
 // DXT1 words from a classified block (as encode_dxt1).
 __device__ __forceinline__ uint2 pack_dxt1(const BlockClass& c)
@@ -364,34 +402,24 @@ struct AlphaRows {
     uint32_t mn, mx;
 };
 
-__device__ __forceinline__ AlphaRows alpha_rows(const uint32_t (&p)[16])
+// Alpha rows of a block whose alpha range [amn, amx] is known.
+__device__ __forceinline__ AlphaRows alpha_rows(const uint32_t (&p)[16], uint32_t amn, uint32_t amx)
 {
     AlphaRows a;
-    // Alpha is the top byte of a pixel word, so the largest (smallest) word
-    // holds the largest (smallest) alpha: v_max3_u32 / v_min3_u32 chains.
-    uint32_t hi = p[0], lo = p[0];
-#pragma unroll
-    for (int i = 1; i < 16; ++i) {
-        hi = max(hi, p[i]);
-        lo = min(lo, p[i]);
-    }
-    a.mx = hi >> 24;
-    a.mn = lo >> 24;
+    a.mn = amn;
+    a.mx = amx;
 #pragma unroll
     for (int r = 0; r < 4; ++r)  // [A0 A1 0 0] | [0 0 A2 A3]
         a.w[r] = perm(p[4 * r + 1], p[4 * r], 0x0c0c0703u) | perm(p[4 * r + 3], p[4 * r + 2], 0x07030c0cu);
     return a;
 }
 
-// From the inverted ascending levels n_k = 255 - lv[k]: ~thr_k in all four
-// bytes, thr_k = ceil((lv[k-1] + lv[k]) / 2).  255 - ceil(y / 2) =
-// floor((510 - y) / 2), the floor average of n_{k-1} and n_k, which is
-// v_lerp_u8 with a zero rounding operand.
-__device__ __forceinline__ void inv_midpoints(const uint32_t (&nlv)[8], uint32_t (&nthr4)[7])
+// From the inverted ascending levels n_k = 255 - lv[k], each replicated to
+// all four bytes: ~thr_k in all four bytes, thr_k = ceil((lv[k-1] + lv[k]) /
+// 2).  255 - ceil(y / 2) = floor((510 - y) / 2), the floor average of n_{k-1}
+// and n_k, which is v_lerp_u8 with a zero rounding operand.
+__device__ __forceinline__ void rep_midpoints(const uint32_t (&r)[8], uint32_t (&nthr4)[7])
 {
-    uint32_t r[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) r[k] = rep4(nlv[k]);
 #pragma unroll
     for (int k = 1; k < 8; ++k) nthr4[k - 1] = __builtin_amdgcn_lerp(r[k - 1], r[k], 0u);
 }
@@ -425,15 +453,19 @@ __device__ __forceinline__ uint32_t sel3(uint32_t s2, uint32_t s1, uint32_t s0)
 // BC3 alpha block as two little-endian words, from the block's alpha rows.
 __device__ __forceinline__ uint2 bc3_alpha(const AlphaRows& a)
 {
-    // level k = mn + floor(k*range/7); (x * 9363) >> 16 == x / 7 for x <= 1785
+    // level k = mn + floor(k*range/7); (x * 9363) >> 16 == x / 7 for x <= 1785.
+    // Inverted: 255 - level = nmn - floor(k*r9 / 2^16) = the high half of
+    // (nmn << 16 | 0xffff) - k*r9 (the low half only absorbs the borrow), so
+    // one multiply-add and one v_perm_b32 of byte 2 give the replicated level.
     const uint32_t range = a.mx - a.mn, r9 = range * 9363u, nmn = 255u - a.mn;
-    uint32_t nlv[8];
-    nlv[0] = nmn;
+    const uint32_t c = (nmn << 16) | 0xFFFFu;
+    uint32_t nlv4[8];
+    nlv4[0] = rep4(nmn);
 #pragma unroll
-    for (int k = 1; k < 7; ++k) nlv[k] = nmn - (((uint32_t)k * r9) >> 16);
-    nlv[7] = 255u - a.mx;
+    for (int k = 1; k < 7; ++k) nlv4[k] = perm(0u, c - (uint32_t)k * r9, 0x02020202u);
+    nlv4[7] = rep4(255u - a.mx);
     uint32_t nthr4[7];
-    inv_midpoints(nlv, nthr4);
+    rep_midpoints(nlv4, nthr4);
     uint32_t g[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -461,8 +493,6 @@ __device__ __forceinline__ uint2 bc3_alpha(const AlphaRows& a)
     return make_uint2(w0, w1);
 }
 
-__device__ __forceinline__ uint2 encode_bc3_alpha(const uint32_t (&p)[16]) { return bc3_alpha(alpha_rows(p)); }
-
 // EAC (ETC2 RGBA8 alpha) block as two little-endian words, from the block's
 // alpha rows.
 __device__ __forceinline__ uint2 eac_alpha(const AlphaRows& a)
@@ -472,13 +502,25 @@ __device__ __forceinline__ uint2 eac_alpha(const AlphaRows& a)
     m = m > 1u ? m : 1u;
     int32_t b = (int32_t)(a.mn + 10u * m);
     if (b > 255) b = (int32_t)a.mx - 9 * (int32_t)m;
-    const int32_t nb = 255 - b, mi = (int32_t)m;
-    const int32_t s[8] = {-10, -8, -5, -2, 1, 4, 7, 9};
-    uint32_t nlv[8];  // 255 - clamp(b + m*s) = clamp(255 - b - m*s)
+    // Inverted levels 255 - clamp(b + m*s) = clamp(nb - m*s), nb = 255 - b,
+    // two to a word in 16-bit lanes.  0 <= b <= 255 (the second base only
+    // occurs for m <= 2 and mx > 235), so for the negative modifiers
+    // nb - m*s >= 0 and only 255 bounds it, and for the positive ones
+    // nb - m*s <= 255 and only 0 does: a v_pk_mad_u16 and one v_pk_min_u16,
+    // or a v_pk_mul_lo_u16 and a saturating v_pk_sub_u16, per pair.
+    const unsigned short nb = (unsigned short)(255 - b), ms = (unsigned short)m;
+    const u16x2 nb2 = {nb, nb}, m2 = {ms, ms}, top = {255, 255};
+    const u16x2 n01 = __builtin_elementwise_min(m2 * u16x2{10, 8} + nb2, top);  // modifiers -10, -8
+    const u16x2 n23 = __builtin_elementwise_min(m2 * u16x2{5, 2} + nb2, top);   // -5, -2
+    const u16x2 n45 = __builtin_elementwise_sub_sat(nb2, m2 * u16x2{1, 4});      // 1, 4
+    const u16x2 n67 = __builtin_elementwise_sub_sat(nb2, m2 * u16x2{7, 9});      // 7, 9
+    const uint32_t w[4] = {__builtin_bit_cast(uint32_t, n01), __builtin_bit_cast(uint32_t, n23),
+                           __builtin_bit_cast(uint32_t, n45), __builtin_bit_cast(uint32_t, n67)};
+    uint32_t nlv4[8];  // level k's byte in all four bytes
 #pragma unroll
-    for (int k = 0; k < 8; ++k) nlv[k] = (uint32_t)min(max(nb - mi * s[k], 0), 255);
+    for (int k = 0; k < 8; ++k) nlv4[k] = perm(0u, w[k >> 1], (k & 1) ? 0x02020202u : 0u);
     uint32_t nthr4[7];
-    inv_midpoints(nlv, nthr4);
+    rep_midpoints(nlv4, nthr4);
     // Selector column groups: even columns (0, 2) and odd ones (1, 3) in the
     // two 16-bit halves, 12 bits each, row 0 in the top three.
     uint32_t ge = 0, go = 0;
@@ -504,17 +546,19 @@ __device__ __forceinline__ uint2 eac_alpha(const AlphaRows& a)
     return make_uint2(perm(hi, head, 0x05060100u), perm(lo, hi, 0x04050600u));
 }
 
-__device__ __forceinline__ uint2 encode_eac_alpha(const uint32_t (&p)[16]) { return eac_alpha(alpha_rows(p)); }
-
-// DXT5 / ETC2 RGBA8 block: {alpha words, colour words}.
+// DXT5 / ETC2 RGBA8 block: {alpha words, colour words}.  One pass over the
+// pixels gives the colour box and the alpha range (block_stats_alpha).
 template <int FMT>
 __device__ __forceinline__ uint4 encode_block16(const uint32_t (&p)[16])
 {
+    uint32_t amn, amx;
+    const BlockStats s = block_stats_alpha(p, amn, amx);
+    const AlphaRows a = alpha_rows(p, amn, amx);
     if constexpr (FMT == kDXT5) {
-        const uint2 al = encode_bc3_alpha(p), c = encode_dxt1(p);
+        const uint2 al = bc3_alpha(a), c = encode_dxt1(p, s);
         return make_uint4(al.x, al.y, c.x, c.y);
     } else {
-        const uint2 al = encode_eac_alpha(p), c = encode_etc1(p);
+        const uint2 al = eac_alpha(a), c = encode_etc1(p, s);
         return make_uint4(al.x, al.y, c.x, c.y);
     }
 }

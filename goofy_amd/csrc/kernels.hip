@@ -245,8 +245,9 @@ __global__ __launch_bounds__(512) void k_encode_dual_alpha(const uint8_t* __rest
     if (first + lane >= nblocks) return;
     uint32_t p[16];
     block_words(v[0], p);
-    const BlockClass c = classify(p);
-    const AlphaRows a = alpha_rows(p);
+    uint32_t amn, amx;
+    const BlockClass c = classify(p, goofy_dev::block_stats_alpha(p, amn, amx));
+    const AlphaRows a = alpha_rows(p, amn, amx);
     const uint2 d = pack_dxt1(c), ba = bc3_alpha(a);
     store16_at<true>(raw_rsrc(dst_dxt5 + (uint64_t)first * 16u), lane * 16u, make_uint4(ba.x, ba.y, d.x, d.y));
     const uint2 e = pack_etc1(p, c), ea = eac_alpha(a);
