@@ -292,12 +292,13 @@ def bench_stress(g, hip, W, launches: int = 20):
 
 # ---- batched --------------------------------------------------------------------------------
 
-def bench_batch(g, hip, W, reps: int = 7, align: int = 16, per_texture: bool = True):
+def bench_batch(g, hip, W, reps: int = 7, align: int = 16, per_texture: bool = True, widths=(0, 1 << 30)):
     """The 4096-texture batch as one hipGraph replay, autotuned shape.  Rows
     are `align`-byte aligned (16: the tight pitch width * 4); with
     per_texture the same textures also run as one plain launch each, for the
-    speedup and a block-for-block comparison."""
-    ts = W.batch_textures()
+    speedup and a block-for-block comparison.  widths = [lo, hi): only the
+    textures of those widths."""
+    ts = [t for t in W.batch_textures() if widths[0] <= t["width"] < widths[1]]
     pitches = [(t["width"] * 4 + align - 1) // align * align for t in ts]
     sizes = [p * t["height"] for p, t in zip(pitches, ts)]
     offs = np.concatenate([[0], np.cumsum([(s + 255) // 256 * 256 for s in sizes])])
@@ -341,7 +342,8 @@ def bench_batch(g, hip, W, reps: int = 7, align: int = 16, per_texture: bool = T
     px = info["pixels"]
     res = {
         "workload": "4096 textures, W,H log-uniform multiples of 4 in [64,2048], 50/50 DXT1/ETC1S, noise U[0,40]"
-                    + ("" if align == 16 else f", rows {align}-byte aligned"),
+                    + ("" if align == 16 else f", rows {align}-byte aligned")
+                    + ("" if widths == (0, 1 << 30) else f", only the {len(ts)} of width in [{widths[0]}, {widths[1]})"),
         "textures": info["textures"], "megapixels": round(px / 1e6, 2),
         "graph_ms": round(graph_ms, 4), "mps": round(px / 1e6 / (graph_ms / 1e3), 1),
         "gbps": round(px / 16 * 72 / (graph_ms / 1e3) / 1e9, 1), "autotuned_config": chosen,
@@ -660,7 +662,8 @@ def main() -> int:
     ap.add_argument("--profile", type=int, default=0, help="rocprof mode: N launches per kernel, no JSON")
     ap.add_argument("--sweep", action="store_true", help="launch-shape sweep table, no JSON")
     ap.add_argument("--batch-align", action="store_true",
-                    help="the batch at tight, 128-B and 256-B aligned row pitches, one JSON object each")
+                    help="the batch at tight, 128-B and 256-B aligned row pitches, then its narrow and wide "
+                         "textures alone; one JSON object each")
     ap.add_argument("--bpl", type=int, default=0)
     ap.add_argument("--wg", type=int, default=0)
     ap.add_argument("--nt", type=int, default=-1)
@@ -686,6 +689,9 @@ def main() -> int:
         for align in (16, 128, 256, 16):
             print(json.dumps({"row_alignment": align, **bench_batch(g, hip, W, reps=15, align=align,
                                                                     per_texture=False)}), flush=True)
+        for widths in ((0, 256), (256, 1 << 30), (1024, 1 << 30)):
+            print(json.dumps({"row_alignment": 16, **bench_batch(g, hip, W, reps=15, per_texture=False,
+                                                                 widths=widths)}), flush=True)
         return 0
 
     cfg = None

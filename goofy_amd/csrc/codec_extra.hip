@@ -18,6 +18,7 @@
 
 #include "block_decode.h"
 #include "internal.h"
+#include "tile_io.h"
 
 #include <mutex>
 
@@ -83,18 +84,30 @@ template <int FMT>
 __global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ blocks, uint8_t* __restrict__ dst,
                                                 uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // a wave's 64 consecutive blocks start at `first` (wave-uniform)
+    const uint32_t first = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
+    const uint32_t lane = threadIdx.x & 63u, i = first + lane;
     if (i >= nblocks) return;
-    const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
     // the four rows are stored together after the block is decoded: storing
     // each row as it comes made the decoders 2-7% slower
     uint32_t px[16];
     decode_block<FMT>(load_block_fmt<FMT>(blocks, i), px);
-    uint8_t* out = dst + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
+    if (tile_near(bw, 64u, pitch)) {
+        const BlockRows t = block_rows(dst, pitch, bw, magic, first);
+        const uint32_t off = block_offset(t, bw, lane);
 #pragma unroll
-    for (int y = 0; y < 4; ++y) {
-        const u32x4 v = {px[4 * y], px[4 * y + 1], px[4 * y + 2], px[4 * y + 3]};
-        __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(out + (uint64_t)y * pitch));
+        for (int y = 0; y < 4; ++y) {
+            const u32x4 v = {px[4 * y], px[4 * y + 1], px[4 * y + 2], px[4 * y + 3]};
+            __builtin_amdgcn_raw_buffer_store_b128(v, t.rows, off, (uint32_t)pitch * y, kBufAux<true>);
+        }
+    } else {
+        const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
+        uint8_t* out = dst + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const u32x4 v = {px[4 * y], px[4 * y + 1], px[4 * y + 2], px[4 * y + 3]};
+            __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(out + (uint64_t)y * pitch));
+        }
     }
 }
 
@@ -192,24 +205,36 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint8_t* __restrict__ block
 {
     ErrSums acc = {};
     const uint64_t step = (uint64_t)gridDim.x * 256u;
-    auto fetch = [&](uint64_t i, uint4& b, u32x4 (&v)[4]) {
-        const uint32_t by = (uint32_t)((i * magic) >> 40), bx = (uint32_t)i - by * bw;
-        const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
-        b = load_block_fmt<FMT>(blocks, i);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            v[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
-    };
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool near = tile_near(bw, 64u, pitch);
     // One block at a time, each decoded row compared as it comes (4 decoded
     // pixels live): at most 64 VGPRs, so registers never limit the waves per
     // SIMD (the grid of at most 1536 workgroups is 6 per SIMD).  A software
     // pipeline that loaded a lane's next block into a second register set
     // before comparing the current one needed 79-82 VGPRs for the ETC formats
     // (5-6 waves per SIMD) and was 1-3% slower.
-    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < nblocks; i += step) {
+    // A wave's 64 lanes take 64 consecutive blocks from `first` (uniform).
+    for (uint64_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u)); w < nblocks;
+         w += step) {
+        const uint32_t first = (uint32_t)w, i = first + lane;
+        if (i >= nblocks) break;
         uint4 b;
         u32x4 v[4];
-        fetch(i, b, v);
+        if (near) {
+            const BlockRows t = block_rows(src, pitch, bw, magic, first);
+            const uint32_t off = block_offset(t, bw, lane);
+            b = load_block_fmt<FMT>(blocks, i);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                v[r] = __builtin_amdgcn_raw_buffer_load_b128(t.rows, off, (uint32_t)pitch * r, kBufAux<true>);
+        } else {
+            const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
+            const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
+            b = load_block_fmt<FMT>(blocks, i);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                v[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
+        }
         decode_block_rows<FMT>(b, [&](int y, const uint32_t (&row)[4]) {
             accum_pixel(v[y].x, row[0], acc);
             accum_pixel(v[y].y, row[1], acc);

@@ -19,6 +19,7 @@
 #include "block_codec.h"
 #include "internal.h"
 #include "synth.h"
+#include "tile_io.h"
 
 namespace goofy_impl {
 
@@ -59,50 +60,24 @@ __device__ __forceinline__ void store16(uint8_t* p, uint4 v)
     else *reinterpret_cast<u32x4*>(p) = w;
 }
 
-// Buffer resource over `base` (raw: no stride, no swizzle; dword 3 as for
-// every gfx9 raw buffer).  Wave-uniform: it lives in four scalar registers.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* base)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0xFFFFFFFF, 0x00020000);
-}
-
-// Cache policy of a buffer access: 2 = nontemporal (streaming), as the "nt"
-// of the global loads and stores above.
-template <bool NT>
-constexpr int kBufAux = NT ? 2 : 0;
-
 // Load the four pixel rows of the blocks first + 64k + lane (k < BPL) of a
-// texture; lanes past nblocks get zeros.
-// Usual case (tile_near): one buffer resource per wave on the first pixel of
-// the tile's first block row; a lane adds a 32-bit byte offset and each row
-// r its wave-uniform r * pitch (the scalar offset of the buffer load), so no
-// lane computes a 64-bit address.  With bw >= 64*BPL a tile spans at most two
-// block rows, and a block of the second one sits 4*pitch + 16*bx < 5*pitch
-// bytes (pitch >= 16*bw) past the first row's start; with pitch < 2^29 the
-// lane offset plus 3 * pitch stays below 2^32.  Narrower textures and larger
-// pitches take a per-lane 64-bit address.
-__device__ __forceinline__ bool tile_near(uint32_t bw, uint32_t bpl, uint64_t pitch)
-{
-    return bw >= 64u * bpl && (pitch >> 29) == 0;
-}
-
+// texture; lanes past nblocks get zeros.  Addressing as in tile_io.h, over
+// runs of 64*BPL blocks; narrower textures and pitches of 2^29 and more take
+// a per-lane 64-bit address.
 template <int BPL, bool NT>
 __device__ __forceinline__ void load_tile(const uint8_t* __restrict__ src, uint64_t pitch, uint32_t bw,
                                           uint64_t magic, uint32_t nblocks, uint32_t first, uint32_t lane,
                                           u32x4 (&px)[BPL][4])
 {
-    if (tile_near(bw, BPL, pitch)) {
-        const uint32_t by0 = (uint32_t)(((uint64_t)first * magic) >> 40), bx0 = first - by0 * bw;
-        const __amdgpu_buffer_rsrc_t rows = raw_rsrc(src + (uint64_t)by0 * 4u * pitch);
-        const uint32_t wrap = (uint32_t)(4u * pitch) - 16u * bw;  // (row + 1, column 0) - (row, column bw)
+    if (tile_near(bw, 64u * BPL, pitch)) {
+        const BlockRows t = block_rows(src, pitch, bw, magic, first);
 #pragma unroll
         for (int k = 0; k < BPL; ++k) {
-            const uint32_t rel = bx0 + 64u * k + lane;
-            const uint32_t off = rel * 16u + (rel >= bw ? wrap : 0u);
+            const uint32_t off = block_offset(t, bw, 64u * k + lane);
             if (first + 64u * k + lane < nblocks) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    px[k][r] = __builtin_amdgcn_raw_buffer_load_b128(rows, off, (uint32_t)pitch * r, kBufAux<NT>);
+                    px[k][r] = __builtin_amdgcn_raw_buffer_load_b128(t.rows, off, (uint32_t)pitch * r, kBufAux<NT>);
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) px[k][r] = u32x4{0u, 0u, 0u, 0u};
