@@ -246,12 +246,17 @@ __device__ __forceinline__ uint32_t etc1_base_bytes(uint32_t avgc, uint32_t mid)
     return base(avgc & 0x00FF00FFu) | ((base((avgc >> 8) & 0x00FF00FFu) & 0xFFu) << 8);
 }
 
-// ETC1 codeword-table index for the brightness range (breakpoints of
-// etc1BrighnessRangeTocontrolByte, GoofyTC/goofy_tc.h:1040-1056).
+// ETC1 codeword-table index for the brightness range: the number of the
+// breakpoints 21, 43, 73, 105, 151, 181, 253 of etc1BrighnessRangeTocontrolByte
+// (GoofyTC/goofy_tc.h:1040-1056) that the range exceeds.  range <= 255, so
+// the seven compares range >= t + 1 run as byte compares, four to a word
+// (avg(~t', r) >= 128 <=> r >= t'), and a popcount of their flags counts them.
 __device__ __forceinline__ uint32_t etc1_table(uint32_t range)
 {
-    return (range > 21u) + (range > 43u) + (range > 73u) + (range > 105u) + (range > 151u) + (range > 181u) +
-           (range > 253u);
+    const uint32_t r4 = rep4(range);
+    const uint32_t lo = avg4(~0x6A4A2C16u, r4);  // t + 1 = 22, 44, 74, 106
+    const uint32_t hi = avg4(~0x00FEB698u, r4);  // 152, 182, 254 (byte 3 unused)
+    return (uint32_t)(__builtin_popcount(lo & 0x80808080u) + __builtin_popcount(hi & 0x00808080u));
 }
 
 // ETC1S block: differential mode, zero deltas, one table for both halves,
