@@ -238,9 +238,12 @@ __device__ __forceinline__ void etc_rows(uint32_t w0, uint32_t w1, Emit&& emit)
 // An alpha palette of 8 levels held as the bytes of {hi:lo}, and the block's
 // 48 code bits; row r's four alphas are one v_perm_b32 with the row's 3-bit
 // codes as selector bytes.
+// An alpha block's palette (level k in byte k of lo | hi << 32) and its 48
+// selector bits in two words, s0 and s1, laid out per format (below) so that
+// a row's four 3-bit selectors come out with a handful of 32-bit operations.
 struct AlphaPalette {
     uint32_t lo, hi;
-    uint64_t bits;
+    uint32_t s0, s1;
 };
 
 // Replace byte 3 (alpha) of a row's four pixels with the bytes of a4.
@@ -251,7 +254,8 @@ __device__ __forceinline__ void put_alpha(uint32_t (&row)[4], uint32_t a4)
 }
 
 // BC3 alpha block (words w0, w1): 8 levels at sevenths when a0 > a1, else 6
-// at fifths plus 0 and 255; codes little-endian, pixel p at bits 3p.
+// at fifths plus 0 and 255; codes little-endian, pixel p at bits 3p of the
+// 48 bits after a0, a1 (row r's four codes at 12r .. 12r + 11).
 __device__ __forceinline__ AlphaPalette bc3_palette(uint32_t w0, uint32_t w1)
 {
     const uint32_t a0 = w0 & 0xFFu, a1 = (w0 >> 8) & 0xFFu;
@@ -270,14 +274,19 @@ __device__ __forceinline__ AlphaPalette bc3_palette(uint32_t w0, uint32_t w1)
     AlphaPalette p;
     p.lo = v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24);
     p.hi = v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24);
-    p.bits = (uint64_t)(w0 >> 16) | ((uint64_t)w1 << 16);
+    p.s0 = (w0 >> 16) | (w1 << 16);  // selector bits 0..31
+    p.s1 = w1 >> 16;                 // bits 32..47
     return p;
 }
 
 __device__ __forceinline__ uint32_t bc3_alpha_row(const AlphaPalette& p, int r)
 {
-    const uint32_t g = (uint32_t)(p.bits >> (12 * r));
-    const uint32_t codes = (g & 7u) | (((g >> 3) & 7u) << 8) | (((g >> 6) & 7u) << 16) | (((g >> 9) & 7u) << 24);
+    // row r's 12 bits at 12r of the 48 (a funnel shift of s1:s0), then the
+    // four 3-bit fields spread to bytes 0..3: fields 0, 1 stay at bits 0..5
+    // and fields 2, 3 move to 16..21, then fields 1 and 3 move up 5 bits
+    const uint32_t g = r < 3 ? __builtin_amdgcn_alignbit(p.s1, p.s0, 12u * (uint32_t)r) : p.s1 >> 4;
+    const uint32_t t = (g & 0x3Fu) | ((g & 0xFC0u) << 10);
+    const uint32_t codes = (t & 0x00070007u) | ((t & 0x00380038u) << 5);
     return __builtin_amdgcn_perm(p.hi, p.lo, codes);
 }
 
@@ -292,7 +301,8 @@ __device__ __forceinline__ uint32_t eac_mags(uint32_t t)
 }
 
 // EAC alpha block (words w0, w1): base, multiplier and table, then 48
-// big-endian selector bits, pixel (x, y) at 45 - 3(4x + y).
+// big-endian selector bits, pixel (x, y) at 45 - 3(4x + y): column c's 12
+// bits at 36 - 12c .. 47 - 12c, row 0 in the top three.
 __device__ __forceinline__ AlphaPalette eac_palette(uint32_t w0, uint32_t w1)
 {
     const int32_t base = (int32_t)(w0 & 0xFFu), mult = (int32_t)((w0 >> 12) & 15u);
@@ -306,15 +316,21 @@ __device__ __forceinline__ AlphaPalette eac_palette(uint32_t w0, uint32_t w1)
         p.lo |= clamp_u8(base - m * mult) << (8 * k);
         p.hi |= clamp_u8(base + (m - 1) * mult) << (8 * k);
     }
-    p.bits = ((uint64_t)__builtin_bswap32(w0 & 0xFFFF0000u) << 32 | __builtin_bswap32(w1)) & 0xFFFFFFFFFFFFull;
+    // s0 / s1: columns 0, 1 / 2, 3 in the two 16-bit lanes, each column's 12
+    // bits as they stand (row r at bit 9 - 3r)
+    const uint32_t hi48 = __builtin_bswap32(w0 & 0xFFFF0000u), lo48 = __builtin_bswap32(w1);  // bits 32..47, 0..31
+    const uint32_t c0 = (hi48 >> 4) & 0xFFFu, c1 = __builtin_amdgcn_alignbit(hi48, lo48, 24u) & 0xFFFu;
+    const uint32_t c2 = (lo48 >> 12) & 0xFFFu, c3 = lo48 & 0xFFFu;
+    p.s0 = c0 | (c1 << 16);
+    p.s1 = c2 | (c3 << 16);
     return p;
 }
 
 __device__ __forceinline__ uint32_t eac_alpha_row(const AlphaPalette& p, int r)
 {
-    uint32_t codes = 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) codes |= (uint32_t)((p.bits >> (45 - 3 * (4 * c + r))) & 7u) << (8 * c);
+    const uint32_t sh = 9u - 3u * (uint32_t)r;
+    const uint32_t x = (p.s0 >> sh) & 0x00070007u, y = (p.s1 >> sh) & 0x00070007u;
+    const uint32_t codes = __builtin_amdgcn_perm(y, x, 0x06040200u);  // [c0, c1, c2, c3]
     return __builtin_amdgcn_perm(p.hi, p.lo, codes);
 }
 

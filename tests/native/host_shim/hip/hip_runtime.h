@@ -64,6 +64,13 @@ inline uint32_t goofy_host_bitop3(uint32_t a, uint32_t b, uint32_t c, uint32_t i
     return out;
 }
 
+// V_ALIGNBIT_B32: bits s..s+31 of the 64-bit value a:b (s taken mod 32).
+inline uint32_t goofy_host_alignbit(uint32_t a, uint32_t b, uint32_t s)
+{
+    return (uint32_t)((((uint64_t)a << 32) | b) >> (s & 31u));
+}
+
 #define __builtin_amdgcn_perm goofy_host_perm
+#define __builtin_amdgcn_alignbit goofy_host_alignbit
 #define __builtin_amdgcn_lerp goofy_host_lerp
 #define __builtin_amdgcn_bitop3_b32 goofy_host_bitop3
