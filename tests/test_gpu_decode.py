@@ -138,6 +138,41 @@ def test_quality_matches_harness_definition(gpu, oracle, fmt):
     assert q["mse_a"] == 0.0 and q["psnr_a"] > 1e300  # alpha is 255 in and out
 
 
+@pytest.mark.parametrize("w,h,pitch", [(200, 84, 832), (1028, 36, 4160), (272, 8, (1 << 29) - 16), (272, 8, 1 << 29)])
+def test_decode_and_quality_address_paths(gpu, oracle, w, h, pitch):
+    """The decode and round-trip kernels on both sides of their addressing
+    limits: 50 blocks a row (under a wave's 64: per-lane 64-bit addresses),
+    257 (one buffer resource per wave, runs straddling two block rows), and
+    68 with pitches just under and at 2^29 bytes, where the wave-uniform base
+    gives way to per-lane addresses (about 3.7 GB each for source and
+    decoded image).  Decoded rows against the oracle's decoder, sums against
+    getMsePsnr restated."""
+    from goofy_amd import hip
+    img = gpu.synth_host(W.WAVES_ALPHA, 5 + w, 0, 20, w, h, w * 4)
+    src = hip.DeviceBuffer(pitch * (h - 1) + w * 4)
+    gpu.synth_device(W.WAVES_ALPHA, 5 + w, 0, 20, src, w, h, pitch)
+    dst = hip.DeviceBuffer(pitch * (h - 1) + w * 4)
+    try:
+        for fmt in (W.DXT1, W.ETC2):
+            n = (w // 4) * (h // 4) * gpu.block_bytes(fmt)
+            blocks = hip.DeviceBuffer(n)
+            print(f"address paths {w}x{h} pitch={pitch} fmt={fmt}", flush=True)
+            gpu.compress_device(fmt, blocks, src, w, h, pitch)
+            enc = blocks.download()
+            assert np.array_equal(enc, oracle.compress_blocks(fmt, img, w, h, w * 4))
+            gpu.decode_device(fmt, dst, blocks, w, h, pitch)
+            hip.synchronize()
+            want = oracle.decode_image(fmt, enc, w, h).reshape(h, w * 4)
+            got = np.stack([dst.download(w * 4, y * pitch) for y in range(h)])
+            assert np.array_equal(got, want)
+            q = gpu.quality_device(fmt, blocks, src, w, h, pitch)
+            assert q["sse_byte"] == reference_quality(img, want)["sse_byte"]
+            blocks.free()
+    finally:
+        src.free()
+        dst.free()
+
+
 @pytest.mark.parametrize("fmt", [W.DXT5, W.ETC2])
 def test_quality_alpha_formats_match_harness(gpu, harness, fmt):
     """Round trip of the alpha image through the 16-byte formats: the GPU
