@@ -54,16 +54,26 @@ __device__ __forceinline__ void bc1_rows(uint32_t w0, uint32_t w1, Emit&& emit)
         const uint32_t r = __builtin_amdgcn_perm(0u, plane[0], sel);
         const uint32_t g = __builtin_amdgcn_perm(0u, plane[1], sel);
         const uint32_t bl = __builtin_amdgcn_perm(0u, plane[2], sel);
-        const uint32_t a = __builtin_amdgcn_perm(0u, alpha, sel);
         const uint32_t rg01 = __builtin_amdgcn_perm(g, r, 0x05010400u);   // R0 G0 R1 G1
         const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);   // R2 G2 R3 G3
-        const uint32_t ba01 = __builtin_amdgcn_perm(a, bl, 0x05010400u);  // B0 A0 B1 A1
-        const uint32_t ba23 = __builtin_amdgcn_perm(a, bl, 0x07030602u);  // B2 A2 B3 A3
-        uint32_t row[4] = {__builtin_amdgcn_perm(ba01, rg01, 0x05040100u),
-                           __builtin_amdgcn_perm(ba01, rg01, 0x07060302u),
-                           __builtin_amdgcn_perm(ba23, rg23, 0x05040100u),
-                           __builtin_amdgcn_perm(ba23, rg23, 0x07060302u)};
-        emit(y, row);
+        if constexpr (FOUR) {
+            // alpha 255 throughout (selector 0x0d), replaced by the caller:
+            // pixel x = R_x G_x from rg, B_x from bl
+            uint32_t row[4] = {__builtin_amdgcn_perm(bl, rg01, 0x0d040100u),
+                               __builtin_amdgcn_perm(bl, rg01, 0x0d050302u),
+                               __builtin_amdgcn_perm(bl, rg23, 0x0d060100u),
+                               __builtin_amdgcn_perm(bl, rg23, 0x0d070302u)};
+            emit(y, row);
+        } else {
+            const uint32_t a = __builtin_amdgcn_perm(0u, alpha, sel);
+            const uint32_t ba01 = __builtin_amdgcn_perm(a, bl, 0x05010400u);  // B0 A0 B1 A1
+            const uint32_t ba23 = __builtin_amdgcn_perm(a, bl, 0x07030602u);  // B2 A2 B3 A3
+            uint32_t row[4] = {__builtin_amdgcn_perm(ba01, rg01, 0x05040100u),
+                               __builtin_amdgcn_perm(ba01, rg01, 0x07060302u),
+                               __builtin_amdgcn_perm(ba23, rg23, 0x05040100u),
+                               __builtin_amdgcn_perm(ba23, rg23, 0x07060302u)};
+            emit(y, row);
+        }
     }
 }
 
@@ -187,19 +197,27 @@ __device__ __forceinline__ void etc_rows(uint32_t w0, uint32_t w1, Emit&& emit)
         }
         return;
     }
-    // plane[c][s]: byte k = clamp(base[s][c] + modifier k), k = selector value
+    // plane[c][s]: byte k = clamp(base[s][c] + modifier k), k = selector value:
+    // [v + a, v + b, v - a, v - b] for the table's {small a, large b}.  In
+    // 16-bit lanes [a, b]: a saturating-at-255 add (v_pk_add_u16 then
+    // v_pk_min_u16), a saturating-at-0 subtract (v_pk_sub_u16 with clamp) and
+    // one v_perm_b32 of the four low bytes.
+    typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
     uint32_t plane[3][2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         const uint32_t cw = (hi >> (5 - 3 * s)) & 7u;
         // ETC1 intensity modifiers {small, large} per table index, one byte each:
         // {2,8},{5,17},{9,29},{13,42},{18,60},{24,80},{33,106},{47,183}
-        const int32_t a = (int32_t)((0x2F2118120D090502ull >> (8 * cw)) & 0xFFu);
-        const int32_t b = (int32_t)((0xB76A503C2A1D1108ull >> (8 * cw)) & 0xFFu);
+        const uint32_t a = (uint32_t)(0x2F2118120D090502ull >> (8 * cw)) & 0xFFu;
+        const uint32_t b = (uint32_t)(0xB76A503C2A1D1108ull >> (8 * cw)) & 0xFFu;
+        const u16x2_t ab = __builtin_bit_cast(u16x2_t, a | (b << 16)), top = {255, 255};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const int32_t v = (int32_t)base[s][c];
-            plane[c][s] = clamp_u8(v + a) | (clamp_u8(v + b) << 8) | (clamp_u8(v - a) << 16) | (clamp_u8(v - b) << 24);
+            const u16x2_t v = __builtin_bit_cast(u16x2_t, base[s][c] * 0x10001u);
+            const uint32_t up = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(v + ab, top));
+            const uint32_t dn = __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(v, ab));
+            plane[c][s] = __builtin_amdgcn_perm(dn, up, 0x06040200u);
         }
     }
     // ETC2: red overflow = T mode, else green = H mode (else blue = planar, below)
@@ -223,12 +241,11 @@ __device__ __forceinline__ void etc_rows(uint32_t w0, uint32_t w1, Emit&& emit)
         const uint32_t bl = __builtin_amdgcn_perm(plane[2][1], plane[2][0], sel);
         const uint32_t rg01 = __builtin_amdgcn_perm(g, r, 0x05010400u);   // R0 G0 R1 G1
         const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);   // R2 G2 R3 G3
-        const uint32_t ba01 = __builtin_amdgcn_perm(0u, bl, 0x0d010d00u); // B0 FF B1 FF
-        const uint32_t ba23 = __builtin_amdgcn_perm(0u, bl, 0x0d030d02u); // B2 FF B3 FF
-        uint32_t row[4] = {__builtin_amdgcn_perm(ba01, rg01, 0x05040100u),
-                           __builtin_amdgcn_perm(ba01, rg01, 0x07060302u),
-                           __builtin_amdgcn_perm(ba23, rg23, 0x05040100u),
-                           __builtin_amdgcn_perm(ba23, rg23, 0x07060302u)};
+        // pixel x = R_x G_x from rg, B_x from bl, alpha 255 (selector 0x0d)
+        uint32_t row[4] = {__builtin_amdgcn_perm(bl, rg01, 0x0d040100u),
+                           __builtin_amdgcn_perm(bl, rg01, 0x0d050302u),
+                           __builtin_amdgcn_perm(bl, rg23, 0x0d060100u),
+                           __builtin_amdgcn_perm(bl, rg23, 0x0d070302u)};
         // ETC2 planar mode (blue overflow alone): the row from O, H and V
         emit(y, row);
     }
