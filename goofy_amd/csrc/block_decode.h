@@ -24,13 +24,20 @@ namespace goofy_dev {
 // four RGBA8 words of row y (y = 0..3, in order), so a caller that consumes
 // rows as they come (a store, a comparison) keeps 4 pixels live, not 16.
 
-// BC1 block rows; FOUR: the colour block of BC3, always in 4-colour mode.
-// Per channel the 4 palette bytes sit in one word (byte k = entry k); a
-// row's pixels come from one v_perm_b32 per channel whose selector bytes are
-// the pixels' 2-bit indices, and four more interleave the channel rows into
-// RGBA words.
-template <bool FOUR, class Emit>
-__device__ __forceinline__ void bc1_rows(uint32_t w0, uint32_t w1, Emit&& emit)
+// Replace byte 3 (alpha) of a row's four pixels with the bytes of a4.
+__device__ __forceinline__ void put_alpha(uint32_t (&row)[4], uint32_t a4)
+{
+#pragma unroll
+    for (int c = 0; c < 4; ++c) row[c] = __builtin_amdgcn_perm(a4, row[c], 0x04020100u + (c << 24));
+}
+
+// BC1 block rows; FOUR: the colour block of BC3, always in 4-colour mode,
+// its alpha bytes from alpha_row(y).  Per channel the 4 palette bytes sit in
+// one word (byte k = entry k); a row's pixels come from one v_perm_b32 per
+// channel whose selector bytes are the pixels' 2-bit indices, and six more
+// interleave the channel rows and the alpha row into RGBA words.
+template <bool FOUR, class Alpha, class Emit>
+__device__ __forceinline__ void bc1_rows(uint32_t w0, uint32_t w1, Alpha&& alpha_row, Emit&& emit)
 {
     const uint32_t c0 = w0 & 0xFFFFu, c1 = w0 >> 16;
     const uint32_t r0 = (c0 >> 11) & 31u, g0 = (c0 >> 5) & 63u, b0 = c0 & 31u;
@@ -56,24 +63,15 @@ __device__ __forceinline__ void bc1_rows(uint32_t w0, uint32_t w1, Emit&& emit)
         const uint32_t bl = __builtin_amdgcn_perm(0u, plane[2], sel);
         const uint32_t rg01 = __builtin_amdgcn_perm(g, r, 0x05010400u);   // R0 G0 R1 G1
         const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);   // R2 G2 R3 G3
-        if constexpr (FOUR) {
-            // alpha 255 throughout (selector 0x0d), replaced by the caller:
-            // pixel x = R_x G_x from rg, B_x from bl
-            uint32_t row[4] = {__builtin_amdgcn_perm(bl, rg01, 0x0d040100u),
-                               __builtin_amdgcn_perm(bl, rg01, 0x0d050302u),
-                               __builtin_amdgcn_perm(bl, rg23, 0x0d060100u),
-                               __builtin_amdgcn_perm(bl, rg23, 0x0d070302u)};
-            emit(y, row);
-        } else {
-            const uint32_t a = __builtin_amdgcn_perm(0u, alpha, sel);
-            const uint32_t ba01 = __builtin_amdgcn_perm(a, bl, 0x05010400u);  // B0 A0 B1 A1
-            const uint32_t ba23 = __builtin_amdgcn_perm(a, bl, 0x07030602u);  // B2 A2 B3 A3
-            uint32_t row[4] = {__builtin_amdgcn_perm(ba01, rg01, 0x05040100u),
-                               __builtin_amdgcn_perm(ba01, rg01, 0x07060302u),
-                               __builtin_amdgcn_perm(ba23, rg23, 0x05040100u),
-                               __builtin_amdgcn_perm(ba23, rg23, 0x07060302u)};
-            emit(y, row);
-        }
+        // alpha: the BC3 alpha block's row, or BC1's own (3-colour mode)
+        const uint32_t a = FOUR ? alpha_row(y) : __builtin_amdgcn_perm(0u, alpha, sel);
+        const uint32_t ba01 = __builtin_amdgcn_perm(a, bl, 0x05010400u);  // B0 A0 B1 A1
+        const uint32_t ba23 = __builtin_amdgcn_perm(a, bl, 0x07030602u);  // B2 A2 B3 A3
+        uint32_t row[4] = {__builtin_amdgcn_perm(ba01, rg01, 0x05040100u),
+                           __builtin_amdgcn_perm(ba01, rg01, 0x07060302u),
+                           __builtin_amdgcn_perm(ba23, rg23, 0x05040100u),
+                           __builtin_amdgcn_perm(ba23, rg23, 0x07060302u)};
+        emit(y, row);
     }
 }
 
@@ -162,8 +160,8 @@ __device__ __forceinline__ void etc2_planar_row(uint32_t hi, uint32_t lo, int y,
 // v_perm_b32 interleave the channel rows into RGBA words (alpha 255 from
 // selector byte 0x0d).  T and H blocks use the same lookup with their four
 // paint colours in both words; planar blocks replace each row's pixels.
-template <class Emit>
-__device__ __forceinline__ void etc_rows(uint32_t w0, uint32_t w1, Emit&& emit)
+template <bool ALPHA, class Alpha, class Emit>
+__device__ __forceinline__ void etc_rows(uint32_t w0, uint32_t w1, Alpha&& alpha_row, Emit&& emit)
 {
     const uint32_t hi = __builtin_bswap32(w0), lo = __builtin_bswap32(w1);  // big-endian words
     const bool diff = (hi >> 1) & 1u, flip = hi & 1u;
@@ -193,6 +191,7 @@ __device__ __forceinline__ void etc_rows(uint32_t w0, uint32_t w1, Emit&& emit)
         for (int y = 0; y < 4; ++y) {
             uint32_t row[4];
             etc2_planar_row(hi, lo, y, row);
+            if constexpr (ALPHA) put_alpha(row, alpha_row(y));
             emit(y, row);
         }
         return;
@@ -208,10 +207,11 @@ __device__ __forceinline__ void etc_rows(uint32_t w0, uint32_t w1, Emit&& emit)
     for (int s = 0; s < 2; ++s) {
         const uint32_t cw = (hi >> (5 - 3 * s)) & 7u;
         // ETC1 intensity modifiers {small, large} per table index, one byte each:
-        // {2,8},{5,17},{9,29},{13,42},{18,60},{24,80},{33,106},{47,183}
-        const uint32_t a = (uint32_t)(0x2F2118120D090502ull >> (8 * cw)) & 0xFFu;
-        const uint32_t b = (uint32_t)(0xB76A503C2A1D1108ull >> (8 * cw)) & 0xFFu;
-        const u16x2_t ab = __builtin_bit_cast(u16x2_t, a | (b << 16)), top = {255, 255};
+        // {2,8},{5,17},{9,29},{13,42},{18,60},{24,80},{33,106},{47,183}; byte cw
+        // of each 8-byte table by v_perm_b32, a into byte 0 and b into byte 2
+        const uint32_t a = __builtin_amdgcn_perm(0x2F211812u, 0x0D090502u, 0x0c0c0c00u | cw);
+        const uint32_t b = __builtin_amdgcn_perm(0xB76A503Cu, 0x2A1D1108u, 0x0c000c0cu | (cw << 16));
+        const u16x2_t ab = __builtin_bit_cast(u16x2_t, a | b), top = {255, 255};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const u16x2_t v = __builtin_bit_cast(u16x2_t, base[s][c] * 0x10001u);
@@ -227,27 +227,41 @@ __device__ __forceinline__ void etc_rows(uint32_t w0, uint32_t w1, Emit&& emit)
 #pragma unroll
         for (int c = 0; c < 3; ++c) plane[c][0] = plane[c][1] = th[c];
     }
+    // Selector bits of pixel (x, y) at k = 4x + y: LSB plane bit k, MSB plane
+    // bit 16 + k, so column x's LSB and MSB nibbles sit at 4x and 16 + 4x.
+    // One byte per column: [LSB nibble | MSB nibble << 4] in byte x (a byte
+    // gather, then the middle nibbles of each 16-bit lane swapped); row y's
+    // selector of column x is then bits y and 4 + y of byte x.
+    const uint32_t g = __builtin_amdgcn_perm(lo, lo, 0x03010200u);  // [L0 L1], [M0 M1], [L2 L3], [M2 M3]
+    const uint32_t cols = (g & 0xF00FF00Fu) | ((g & 0x00F000F0u) << 4) | ((g >> 4) & 0x00F000F0u);
     // sub-block 1: rows 2..3 (flip) or columns 2..3
 #pragma unroll
     for (int y = 0; y < 4; ++y) {
-        // selector bits of pixel (x, y) at k = 4x + y: LSB plane bit k, MSB plane bit 16 + k
-        const uint32_t l = (lo >> y) & 0x1111u, m = (lo >> (16 + y)) & 0x1111u;
-        const uint32_t v = l | (m << 1);  // pixel x's selector in bits 4x..4x+1
-        const uint32_t w = (v & 0xFFu) | ((v & 0xFF00u) << 8);
-        uint32_t sel = (w & 0x000F000Fu) | ((w & 0x00F000F0u) << 4);  // pixel x's selector in byte x
+        const uint32_t t = cols >> y;
+        uint32_t sel = (t & 0x01010101u) | ((t >> 3) & 0x02020202u);  // pixel x's selector in byte x
         sel |= flip ? (y >= 2 ? 0x04040404u : 0u) : 0x04040000u;
         const uint32_t r = __builtin_amdgcn_perm(plane[0][1], plane[0][0], sel);
         const uint32_t g = __builtin_amdgcn_perm(plane[1][1], plane[1][0], sel);
         const uint32_t bl = __builtin_amdgcn_perm(plane[2][1], plane[2][0], sel);
         const uint32_t rg01 = __builtin_amdgcn_perm(g, r, 0x05010400u);   // R0 G0 R1 G1
         const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);   // R2 G2 R3 G3
-        // pixel x = R_x G_x from rg, B_x from bl, alpha 255 (selector 0x0d)
-        uint32_t row[4] = {__builtin_amdgcn_perm(bl, rg01, 0x0d040100u),
-                           __builtin_amdgcn_perm(bl, rg01, 0x0d050302u),
-                           __builtin_amdgcn_perm(bl, rg23, 0x0d060100u),
-                           __builtin_amdgcn_perm(bl, rg23, 0x0d070302u)};
-        // ETC2 planar mode (blue overflow alone): the row from O, H and V
-        emit(y, row);
+        if constexpr (ALPHA) {
+            const uint32_t a = alpha_row(y);
+            const uint32_t ba01 = __builtin_amdgcn_perm(a, bl, 0x05010400u);  // B0 A0 B1 A1
+            const uint32_t ba23 = __builtin_amdgcn_perm(a, bl, 0x07030602u);  // B2 A2 B3 A3
+            uint32_t row[4] = {__builtin_amdgcn_perm(ba01, rg01, 0x05040100u),
+                               __builtin_amdgcn_perm(ba01, rg01, 0x07060302u),
+                               __builtin_amdgcn_perm(ba23, rg23, 0x05040100u),
+                               __builtin_amdgcn_perm(ba23, rg23, 0x07060302u)};
+            emit(y, row);
+        } else {
+            // pixel x = R_x G_x from rg, B_x from bl, alpha 255 (selector 0x0d)
+            uint32_t row[4] = {__builtin_amdgcn_perm(bl, rg01, 0x0d040100u),
+                               __builtin_amdgcn_perm(bl, rg01, 0x0d050302u),
+                               __builtin_amdgcn_perm(bl, rg23, 0x0d060100u),
+                               __builtin_amdgcn_perm(bl, rg23, 0x0d070302u)};
+            emit(y, row);
+        }
     }
 }
 
@@ -262,13 +276,6 @@ struct AlphaPalette {
     uint32_t lo, hi;
     uint32_t s0, s1;
 };
-
-// Replace byte 3 (alpha) of a row's four pixels with the bytes of a4.
-__device__ __forceinline__ void put_alpha(uint32_t (&row)[4], uint32_t a4)
-{
-#pragma unroll
-    for (int c = 0; c < 4; ++c) row[c] = __builtin_amdgcn_perm(a4, row[c], 0x04020100u + (c << 24));
-}
 
 // BC3 alpha block (words w0, w1): 8 levels at sevenths when a0 > a1, else 6
 // at fifths plus 0 and 255; codes little-endian, pixel p at bits 3p of the
@@ -356,22 +363,17 @@ __device__ __forceinline__ uint32_t eac_alpha_row(const AlphaPalette& p, int r)
 template <int FMT, class Emit>
 __device__ __forceinline__ void decode_block_rows(uint4 b, Emit&& emit)
 {
+    const auto opaque = [](int) { return 0xFFFFFFFFu; };
     if constexpr (FMT == 0) {
-        bc1_rows<false>(b.x, b.y, emit);
+        bc1_rows<false>(b.x, b.y, opaque, emit);
     } else if constexpr (FMT == 1) {
-        etc_rows(b.x, b.y, emit);
+        etc_rows<false>(b.x, b.y, opaque, emit);
     } else if constexpr (FMT == 2) {
         const AlphaPalette p = bc3_palette(b.x, b.y);
-        bc1_rows<true>(b.z, b.w, [&](int y, uint32_t (&row)[4]) {
-            put_alpha(row, bc3_alpha_row(p, y));
-            emit(y, row);
-        });
+        bc1_rows<true>(b.z, b.w, [&](int y) { return bc3_alpha_row(p, y); }, emit);
     } else {
         const AlphaPalette p = eac_palette(b.x, b.y);
-        etc_rows(b.z, b.w, [&](int y, uint32_t (&row)[4]) {
-            put_alpha(row, eac_alpha_row(p, y));
-            emit(y, row);
-        });
+        etc_rows<true>(b.z, b.w, [&](int y) { return eac_alpha_row(p, y); }, emit);
     }
 }
 
