@@ -40,6 +40,15 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96u);
 }
 
+// Bits of m where `sel` is set, bits of o elsewhere: one v_bitop3_b32
+// (truth table indexed by sel*4 + m*2 + o).  Written as the builtin so that
+// the compiler keeps the single select instead of re-deriving masks from
+// the known-zero bits of shifted operands.
+__device__ __forceinline__ uint32_t bsel(uint32_t sel, uint32_t m, uint32_t o)
+{
+    return __builtin_amdgcn_bitop3_b32(sel, m, o, 0xCAu);
+}
+
 // v_perm_b32: byte i of the result is byte sel[i] of {hi:lo} (0..3 = lo,
 // 4..7 = hi, 0x0c = zero).
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
@@ -54,6 +63,29 @@ __device__ __forceinline__ uint32_t pk_min16(uint32_t a, uint32_t b)
 __device__ __forceinline__ uint32_t pk_max16(uint32_t a, uint32_t b)
 {
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+
+// Three-input min / max of 16-bit lanes that hold 0..255, as one
+// v_pk_minimum3_f16 / v_pk_maximum3_f16: read as half-precision numbers such
+// lanes are +0 and positive subnormals, whose order is that of the integers,
+// and the kernels keep half-precision subnormals (denorm mode 16/64 = 3), so
+// the selected lane comes out bit for bit.  One instruction takes two new
+// pairs of lanes where v_pk_min_u16 takes one.
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t min3_b16(uint32_t a, uint32_t b, uint32_t c)
+{
+    const h16x2 r = __builtin_elementwise_minimum(
+        __builtin_elementwise_minimum(__builtin_bit_cast(h16x2, a), __builtin_bit_cast(h16x2, b)),
+        __builtin_bit_cast(h16x2, c));
+    return __builtin_bit_cast(uint32_t, r);
+}
+__device__ __forceinline__ uint32_t max3_b16(uint32_t a, uint32_t b, uint32_t c)
+{
+    const h16x2 r = __builtin_elementwise_maximum(
+        __builtin_elementwise_maximum(__builtin_bit_cast(h16x2, a), __builtin_bit_cast(h16x2, b)),
+        __builtin_bit_cast(h16x2, c));
+    return __builtin_bit_cast(uint32_t, r);
 }
 
 // Luma of the colour in byte lanes 0..2 of c (result in byte 0, other bytes
@@ -111,32 +143,41 @@ __device__ __forceinline__ BlockStats corner_stats(uint32_t mnrb, uint32_t mxrb,
     return s;
 }
 
-// R, B bounding box in the two 16-bit halves of p & 0x00ff00ff.
+// R, B bounding box in the two 16-bit halves of p & 0x00ff00ff: the 16
+// masked words through a chain of three-input min / max (8 + 8 instructions
+// instead of 15 + 15).
 __device__ __forceinline__ void rb_box(const uint32_t (&p)[16], uint32_t& mnrb, uint32_t& mxrb)
 {
-    mnrb = p[0] & 0x00FF00FFu;
-    mxrb = mnrb;
+    uint32_t rb[16];
 #pragma unroll
-    for (int i = 1; i < 16; ++i) {
-        const uint32_t rb = p[i] & 0x00FF00FFu;
-        mnrb = pk_min16(mnrb, rb);
-        mxrb = pk_max16(mxrb, rb);
+    for (int i = 0; i < 16; ++i) rb[i] = p[i] & 0x00FF00FFu;
+    mnrb = min3_b16(rb[0], rb[1], rb[2]);
+    mxrb = max3_b16(rb[0], rb[1], rb[2]);
+#pragma unroll
+    for (int i = 3; i < 15; i += 2) {
+        mnrb = min3_b16(mnrb, rb[i], rb[i + 1]);
+        mxrb = max3_b16(mxrb, rb[i], rb[i + 1]);
     }
+    mnrb = pk_min16(mnrb, rb[15]);
+    mxrb = pk_max16(mxrb, rb[15]);
 }
 
 __device__ __forceinline__ BlockStats block_stats(const uint32_t (&p)[16])
 {
     uint32_t mnrb, mxrb;
     rb_box(p, mnrb, mxrb);
-    // G of two pixels packed into one word with v_perm ([G_a 0 G_b 0])
-    uint32_t g = perm(p[1], p[0], 0x0c050c01u);
-    uint32_t mng = g, mxg = g;
+    // G of two pixels packed into one word with v_perm ([G_a 0 G_b 0]),
+    // then the three-input min / max over the eight words
+    uint32_t g[8];
 #pragma unroll
-    for (int i = 2; i < 16; i += 2) {
-        g = perm(p[i + 1], p[i], 0x0c050c01u);
-        mng = pk_min16(mng, g);
-        mxg = pk_max16(mxg, g);
-    }
+    for (int i = 0; i < 8; ++i) g[i] = perm(p[2 * i + 1], p[2 * i], 0x0c050c01u);
+    uint32_t mng = min3_b16(g[0], g[1], g[2]), mxg = max3_b16(g[0], g[1], g[2]);
+    mng = min3_b16(mng, g[3], g[4]);
+    mxg = max3_b16(mxg, g[3], g[4]);
+    mng = min3_b16(mng, g[5], g[6]);
+    mxg = max3_b16(mxg, g[5], g[6]);
+    mng = pk_min16(mng, g[7]);
+    mxg = pk_max16(mxg, g[7]);
     mng = pk_min16(mng, mng >> 16) & 0xFFu;
     mxg = pk_max16(mxg, mxg >> 16) & 0xFFu;
     return corner_stats(mnrb, mxrb, mng, mxg);
@@ -192,6 +233,39 @@ __device__ __forceinline__ uint32_t dxt1_endpoints(const BlockStats& s)
     return (r << 11) | (g << 6) | b | 0x20u;
 }
 
+// The DXT1 index word from the per-row compare masks (bit 7 of byte c of
+// gez[r] / near[r] for pixel (r, c)): pixel (r, c)'s 2-bit code, bit 0 =
+// y < mid and bit 1 = near, at bits 8r + 2c.  Per row, bit selects put both
+// flags in bits 6, 7 of each byte, the byte pairs' codes side by side in bits
+// 6..9 and 22..25, and all four in bits 6..13.
+__device__ __forceinline__ uint32_t dxt1_index_word(const uint32_t (&gez)[4], const uint32_t (&near)[4])
+{
+    uint32_t idx = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        // near in bit 7, ~gez in bit 6 (truth table 0xC5: sel ? near : ~gez)
+        const uint32_t z = __builtin_amdgcn_bitop3_b32(0x80808080u, near[r], gez[r] >> 1, 0xC5u);
+        const uint32_t y = bsel(0x00C000C0u, z, z >> 6);
+        const uint32_t g = bsel(0x000003C0u, y, y >> 12);
+        idx |= ((g >> 6) & 0xFFu) << (8 * r);
+    }
+    return idx;
+}
+
+// The ETC1S selector word (goofy_tc.h:1375-1395: word bit j <-> column
+// {2,3,0,1}[j>>2], row j&3; the MSB plane N in bits 16..31, then inverted)
+// from the same masks.  Each plane gathers row r's flags into bit 4 + r of
+// byte c (row 3's stay in bit 7), then a bit select folds the column nibbles
+// into bytes 1 (columns 0, 1) and 3 (columns 2, 3).
+__device__ __forceinline__ uint32_t etc1_selector_word(const uint32_t (&gez)[4], const uint32_t (&near)[4])
+{
+    auto plane = [](const uint32_t (&f)[4]) {
+        const uint32_t t = bsel(0x10101010u, f[0] >> 3, bsel(0x20202020u, f[1] >> 2, bsel(0x40404040u, f[2] >> 1, f[3])));
+        return bsel(0x0F000F00u, t << 4, t);
+    };
+    return ~perm(plane(near), plane(gez), 0x05070103u);  // G: cols 2|3, 0|1; then N's
+}
+
 // DXT1 (BC1) block: returns {c0 | c1<<16, indices} as two little-endian words.
 __device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16], const BlockStats& s)
 {
@@ -201,20 +275,11 @@ __device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16], const Bloc
     const uint32_t nlo4 = rep4(~lo);
     const uint32_t hi4 = rep4(hi);
 
-    // code byte per pixel: bit0 = y < mid, bit1 = near; row r in word r
-    uint32_t idx = 0;
+    uint32_t gez[4], near[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t y4 = luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]);
-        uint32_t gez, near;
-        compare4(y4, nmid4, nlo4, hi4, &gez, &near);
-        const uint32_t code = ((~gez >> 7) & 0x01010101u) | ((near >> 6) & 0x02020202u);
-        // bytes (c0,c1,c2,c3) with 2-bit codes -> one byte c0|c1<<2|c2<<4|c3<<6
-        uint32_t t = code | (code >> 6);
-        t = t | (t >> 12);
-        idx |= (t & 0xFFu) << (8 * r);
-    }
-    return make_uint2(dxt1_endpoints(s), idx);
+    for (int r = 0; r < 4; ++r)
+        compare4(luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]), nmid4, nlo4, hi4, &gez[r], &near[r]);
+    return make_uint2(dxt1_endpoints(s), dxt1_index_word(gez, near));
 }
 
 __device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16]) { return encode_dxt1(p, block_stats(p)); }
@@ -269,23 +334,11 @@ __device__ __forceinline__ uint2 encode_etc1(const uint32_t (&p)[16], const Bloc
     const uint32_t nlo4 = rep4(~lo);
     const uint32_t hi4 = rep4(hi);
 
-    // Selector planes.  Row r's four MSB flags (bit 8c+7) go to bit 8c+r, so
-    // byte c of G/N holds column c's nibble (rows 0..3).
-    uint32_t G = 0, N = 0;
+    uint32_t gez[4], near[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t y4 = luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]);
-        uint32_t gez, near;
-        compare4(y4, nmid4, nlo4, hi4, &gez, &near);
-        G |= (gez & 0x80808080u) >> (7 - r);
-        N |= (near & 0x80808080u) >> (7 - r);
-    }
-    // Word bit j <-> column {2,3,0,1}[j>>2], row j&3 (goofy_tc.h:1375-1395):
-    // fold nibbles (byte0 = col0|col1<<4, byte2 = col2|col3<<4), then put
-    // G's pair in bits 0..15 and N's in 16..31 and invert.
-    G |= G >> 4;
-    N |= N >> 4;
-    const uint32_t w1 = ~perm(N, G, 0x04060002u);
+    for (int r = 0; r < 4; ++r)
+        compare4(luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]), nmid4, nlo4, hi4, &gez[r], &near[r]);
+    const uint32_t w1 = etc1_selector_word(gez, near);
 
     // Average colour: rows first, then columns (goofy_tc.h:1402-1414).
     uint32_t col[4];
@@ -335,36 +388,13 @@ __device__ __forceinline__ BlockClass classify(const uint32_t (&p)[16]) { return
 // DXT1 words from a classified block (as encode_dxt1).
 __device__ __forceinline__ uint2 pack_dxt1(const BlockClass& c)
 {
-    // code byte per pixel: bit0 = y < mid, bit1 = near; row r in word r
-    uint32_t idx = 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t code = ((~c.gez[r] >> 7) & 0x01010101u) | ((c.near[r] >> 6) & 0x02020202u);
-        // bytes (c0,c1,c2,c3) with 2-bit codes -> one byte c0|c1<<2|c2<<4|c3<<6
-        uint32_t t = code | (code >> 6);
-        t = t | (t >> 12);
-        idx |= (t & 0xFFu) << (8 * r);
-    }
-    return make_uint2(dxt1_endpoints(c.s), idx);
+    return make_uint2(dxt1_endpoints(c.s), dxt1_index_word(c.gez, c.near));
 }
 
 // ETC1S words from a classified block and its pixels (as encode_etc1).
 __device__ __forceinline__ uint2 pack_etc1(const uint32_t (&p)[16], const BlockClass& c)
 {
-    // Selector planes.  Row r's four MSB flags (bit 8c+7) go to bit 8c+r, so
-    // byte c of G/N holds column c's nibble (rows 0..3).
-    uint32_t G = 0, N = 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        G |= (c.gez[r] & 0x80808080u) >> (7 - r);
-        N |= (c.near[r] & 0x80808080u) >> (7 - r);
-    }
-    // Word bit j <-> column {2,3,0,1}[j>>2], row j&3 (goofy_tc.h:1375-1395):
-    // fold nibbles (byte0 = col0|col1<<4, byte2 = col2|col3<<4), then put
-    // G's pair in bits 0..15 and N's in 16..31 and invert.
-    G |= G >> 4;
-    N |= N >> 4;
-    const uint32_t w1 = ~perm(N, G, 0x04060002u);
+    const uint32_t w1 = etc1_selector_word(c.gez, c.near);
 
     // Average colour: rows first, then columns (goofy_tc.h:1402-1414).
     uint32_t col[4];
@@ -437,15 +467,6 @@ __device__ __forceinline__ void rank_flags(uint32_t w, const uint32_t (&nthr4)[7
 {
 #pragma unroll
     for (int k = 0; k < 7; ++k) m[k] = avg4(nthr4[k], w);  // avg(~t, v) >= 128 <=> v >= t
-}
-
-// Bits of m where `sel` is set, bits of o elsewhere: one v_bitop3_b32
-// (truth table indexed by sel*4 + m*2 + o).  Written as the builtin so that
-// the compiler keeps the single select instead of re-deriving masks from
-// the known-zero bits of shifted operands.
-__device__ __forceinline__ uint32_t bsel(uint32_t sel, uint32_t m, uint32_t o)
-{
-    return __builtin_amdgcn_bitop3_b32(sel, m, o, 0xCAu);
 }
 
 // Three selector bits at bit 7 of each byte -> the 3-bit selector in bits
