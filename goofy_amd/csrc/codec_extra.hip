@@ -192,56 +192,112 @@ __device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], uns
     }
 }
 
+// One wave's tile of the round trip: 64 consecutive blocks from `first`,
+// each lane's block and its four source rows.  Lanes past the last block
+// load the last block again and skip its sums, so every load is issued
+// unconditionally.
+template <int FMT>
+struct QTile {
+    uint4 b;
+    u32x4 v[4];
+};
+
+template <int FMT, bool NEAR>
+__device__ __forceinline__ void load_qtile(const uint8_t* __restrict__ blocks, const uint8_t* __restrict__ src,
+                                           uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
+                                           uint32_t first, uint32_t lane, QTile<FMT>& t)
+{
+    const uint32_t k = min(lane, nblocks - 1u - first), i = first + k;
+    t.b = load_block_fmt<FMT>(blocks, i);
+    if constexpr (NEAR) {
+        const BlockRows r = block_rows(src, pitch, bw, magic, first);
+        const uint32_t off = block_offset(r, bw, k);
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+            t.v[y] = __builtin_amdgcn_raw_buffer_load_b128(r.rows, off, (uint32_t)pitch * y, kBufAux<true>);
+    } else {
+        const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
+        const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+            t.v[y] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)y * pitch));
+    }
+}
+
+// Decode the lane's block and add its errors against the source rows, each
+// decoded row compared as it comes (4 decoded pixels live).
+template <int FMT>
+__device__ __forceinline__ void sqerr_tile(const QTile<FMT>& t, bool valid, ErrSums& acc)
+{
+    if (valid)
+        decode_block_rows<FMT>(t.b, [&](int y, const uint32_t (&row)[4]) {
+            accum_pixel(t.v[y].x, row[0], acc);
+            accum_pixel(t.v[y].y, row[1], acc);
+            accum_pixel(t.v[y].z, row[2], acc);
+            accum_pixel(t.v[y].w, row[3], acc);
+        });
+}
+
+// A wave's tiles first, first + step, ... in a grid-stride loop.
+// PIPELINE (BC1, BC3): two register sets in turn, the next tile's loads
+// issued before the current tile is compared; every path into a comparison
+// has the same loads outstanding after it, so its wait leaves the other
+// set's loads in flight (65 / 73 VGPRs, within the grid's 6 waves per SIMD;
+// quality calls 2-2.5% faster).  Otherwise one tile at a time (59 / 61
+// VGPRs for the ETC formats, whose two-set form needed 78-85 and waited for
+// both sets at every decode).
+template <int FMT, bool NEAR, bool PIPELINE>
+__device__ __forceinline__ void sqerr_loop(const uint8_t* __restrict__ blocks, const uint8_t* __restrict__ src,
+                                           uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
+                                           ErrSums& acc)
+{
+    const uint32_t step = gridDim.x * 256u;  // nblocks <= kQualityMaxLaunchUnits < 2^30: no overflow below
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u));
+    if (t >= nblocks) return;
+    QTile<FMT> a, b;
+    if constexpr (PIPELINE) {
+        uint32_t n = (nblocks - 1u - t) / step + 1u;  // tiles of this wave; all but the last are whole
+        load_qtile<FMT, NEAR>(blocks, src, pitch, bw, nblocks, magic, t, lane, a);
+        while (n >= 3u) {
+            load_qtile<FMT, NEAR>(blocks, src, pitch, bw, nblocks, magic, t + step, lane, b);
+            sqerr_tile<FMT>(a, true, acc);
+            load_qtile<FMT, NEAR>(blocks, src, pitch, bw, nblocks, magic, t + 2u * step, lane, a);
+            sqerr_tile<FMT>(b, true, acc);
+            t += 2u * step;
+            n -= 2u;
+        }
+        if (n == 2u) {
+            load_qtile<FMT, NEAR>(blocks, src, pitch, bw, nblocks, magic, t + step, lane, b);
+            sqerr_tile<FMT>(a, true, acc);
+            sqerr_tile<FMT>(b, t + step + lane < nblocks, acc);
+        } else {
+            sqerr_tile<FMT>(a, t + lane < nblocks, acc);
+        }
+    } else {
+        for (; t < nblocks; t += step) {
+            load_qtile<FMT, NEAR>(blocks, src, pitch, bw, nblocks, magic, t, lane, a);
+            sqerr_tile<FMT>(a, t + lane < nblocks, acc);
+        }
+    }
+}
+
 // Round-trip error: decode each block in registers and compare it with the
 // source pixels.  Per workgroup: e[0..3] = squared error of bytes 0..3 of the
 // pixels, y100 = sum of squared luma differences x 10^4 with the harness's
 // weights (0.21, 0.72, 0.07 on bytes 2, 1, 0 -- getMsePsnr reads a pixel as a
 // little-endian word and takes bits 16..23 as "red").  Grid-stride over the
-// blocks, one partial per workgroup; k_sums_final adds the partials.
+// blocks (at most 1536 workgroups), one partial per workgroup; k_sums_final
+// adds the partials.
 template <int FMT>
 __global__ __launch_bounds__(256) void k_sqerr(const uint8_t* __restrict__ blocks, const uint8_t* __restrict__ src,
                                                uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
                                                QualitySums* __restrict__ part)
 {
+    constexpr bool kPipeline = FMT == 0 || FMT == 2;
     ErrSums acc = {};
-    const uint64_t step = (uint64_t)gridDim.x * 256u;
-    const uint32_t lane = threadIdx.x & 63u;
-    const bool near = tile_near(bw, 64u, pitch);
-    // One block at a time, each decoded row compared as it comes (4 decoded
-    // pixels live): at most 64 VGPRs, so registers never limit the waves per
-    // SIMD (the grid of at most 1536 workgroups is 6 per SIMD).  A software
-    // pipeline that loaded a lane's next block into a second register set
-    // before comparing the current one needed 79-82 VGPRs for the ETC formats
-    // (5-6 waves per SIMD) and was 1-3% slower.
-    // A wave's 64 lanes take 64 consecutive blocks from `first` (uniform).
-    for (uint64_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u)); w < nblocks;
-         w += step) {
-        const uint32_t first = (uint32_t)w, i = first + lane;
-        if (i >= nblocks) break;
-        uint4 b;
-        u32x4 v[4];
-        if (near) {
-            const BlockRows t = block_rows(src, pitch, bw, magic, first);
-            const uint32_t off = block_offset(t, bw, lane);
-            b = load_block_fmt<FMT>(blocks, i);
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                v[r] = __builtin_amdgcn_raw_buffer_load_b128(t.rows, off, (uint32_t)pitch * r, kBufAux<true>);
-        } else {
-            const uint32_t by = (uint32_t)(((uint64_t)i * magic) >> 40), bx = i - by * bw;
-            const uint8_t* row = src + (uint64_t)by * 4u * pitch + (uint64_t)bx * 16u;
-            b = load_block_fmt<FMT>(blocks, i);
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                v[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + (uint64_t)r * pitch));
-        }
-        decode_block_rows<FMT>(b, [&](int y, const uint32_t (&row)[4]) {
-            accum_pixel(v[y].x, row[0], acc);
-            accum_pixel(v[y].y, row[1], acc);
-            accum_pixel(v[y].z, row[2], acc);
-            accum_pixel(v[y].w, row[3], acc);
-        });
-    }
+    if (tile_near(bw, 64u, pitch)) sqerr_loop<FMT, true, kPipeline>(blocks, src, pitch, bw, nblocks, magic, acc);
+    else sqerr_loop<FMT, false, kPipeline>(blocks, src, pitch, bw, nblocks, magic, acc);
     unsigned long long e[4], ey;
     lane_totals(acc, e, ey);
     block_sums_store(e, ey, part + blockIdx.x);
