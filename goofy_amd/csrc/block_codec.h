@@ -114,14 +114,15 @@ __device__ __forceinline__ uint32_t rep4(uint32_t x) { return perm(0u, x, 0x0000
 __device__ __forceinline__ uint32_t q5(uint32_t c) { return (c - (c != 0u ? 1u : 0u)) >> 3; }
 
 struct BlockStats {
-    uint32_t mn, mx;      // packed RGBA bounding-box corners
     uint32_t range, mid, qt;
-    uint32_t mnrb, mxrb;  // R, B of the corners in 16-bit lanes
+    uint32_t mnrb, mxrb;  // R, B of the bounding-box corners in 16-bit lanes
     uint32_t mng, mxg;    // G of the corners
 };
 
 // The rest of the statistics from the bounding-box corners: R, B in the
-// 16-bit lanes of mnrb / mxrb, G in mng / mxg.
+// 16-bit lanes of mnrb / mxrb, G in mng / mxg.  Both corners' lumas come
+// out of one pass of byte averages (bytes 0 and 1 of y2), and the roundings
+// (x + 1) >> 1 of mid and qt are byte averages too (range <= 255).
 __device__ __forceinline__ BlockStats corner_stats(uint32_t mnrb, uint32_t mxrb, uint32_t mng, uint32_t mxg)
 {
     BlockStats s;
@@ -129,16 +130,14 @@ __device__ __forceinline__ BlockStats corner_stats(uint32_t mnrb, uint32_t mxrb,
     s.mxrb = mxrb;
     s.mng = mng;
     s.mxg = mxg;
-    s.mn = mnrb | (mng << 8);
-    s.mx = mxrb | (mxg << 8);
-    const uint32_t ymn = luma_b0(s.mn);
-    const uint32_t ymx = luma_b0(s.mx);  // >= ymn: luma is monotone per channel
-    const uint32_t d = ymx - ymn;
+    const uint32_t rb = perm(mxrb, mnrb, 0x06020400u);             // [R_mn R_mx B_mn B_mx]
+    const uint32_t y2 = avg4(avg4(rb, rb >> 16), mng | (mxg << 8));  // [Y(mn) Y(mx) . .]
+    const uint32_t d = ((y2 >> 8) & 0xFFu) - (y2 & 0xFFu);          // >= 0: luma is monotone per channel
     s.range = d > 8u ? d : 8u;
-    s.mid = (ymn + ymx + 1u) >> 1;
-    const uint32_t half = (s.range + 1u) >> 1;
-    const uint32_t quarter = (half + 1u) >> 1;
-    const uint32_t eighth = (quarter + 1u) >> 1;
+    s.mid = avg4(y2, y2 >> 8) & 0xFFu;
+    const uint32_t half = avg4(s.range, 0u);
+    const uint32_t quarter = avg4(half, 0u);
+    const uint32_t eighth = avg4(quarter, 0u);
     s.qt = quarter + eighth;  // <= 96
     return s;
 }
@@ -205,12 +204,15 @@ __device__ __forceinline__ BlockStats block_stats_alpha(const uint32_t (&p)[16],
 
 // Per-byte comparison masks for four packed lumas y4 against the block's mid
 // and threshold: bit 7 of each byte of *gez = (y >= mid), of *near =
-// (|y - mid| < qt).  Uses avg(~a, b) >= 128  <=>  b >= a for bytes a, b.
-__device__ __forceinline__ void compare4(uint32_t y4, uint32_t nmid4, uint32_t nlo4, uint32_t hi4, uint32_t* gez,
+// (|y - mid| < qt) = (lo <= y <= hi).  Uses avg(~a, b) >= 128  <=>  b >= a
+// for bytes a, b, and for the upper bound the floor average:
+// (~hi + y) >> 1 >= 128  <=>  y > hi, which also holds for hi = 255 (never).
+// nhi4 = ~hi in all four bytes.
+__device__ __forceinline__ void compare4(uint32_t y4, uint32_t nmid4, uint32_t nlo4, uint32_t nhi4, uint32_t* gez,
                                          uint32_t* near)
 {
     *gez = avg4(nmid4, y4);
-    *near = avg4(nlo4, y4) & avg4(~y4, hi4);
+    *near = avg4(nlo4, y4) & ~__builtin_amdgcn_lerp(nhi4, y4, 0u);
 }
 
 // q5 of the two 16-bit lanes of x: subsat(c, 1) >> 3 per lane.
@@ -273,40 +275,39 @@ __device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16], const Bloc
     const uint32_t hi = s.mid + s.qt - 1u < 255u ? s.mid + s.qt - 1u : 255u;  // y <= hi
     const uint32_t nmid4 = rep4(~s.mid);
     const uint32_t nlo4 = rep4(~lo);
-    const uint32_t hi4 = rep4(hi);
+    const uint32_t nhi4 = rep4(~hi);
 
     uint32_t gez[4], near[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-        compare4(luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]), nmid4, nlo4, hi4, &gez[r], &near[r]);
+        compare4(luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]), nmid4, nlo4, nhi4, &gez[r], &near[r]);
     return make_uint2(dxt1_endpoints(s), dxt1_index_word(gez, near));
 }
 
 __device__ __forceinline__ uint2 encode_dxt1(const uint32_t (&p)[16]) { return encode_dxt1(p, block_stats(p)); }
 
-// ETC1 table index and base-colour bytes of the block, from its average
-// colour: each channel of avgc moved by the luma correction (up: + corr,
-// else - corr) with saturation to 0..255 (goofy_tc.h:1431-1449), then cut to
-// 5 bits in place, q5(b) << 3 == subsat(b, 1) & 0xf8.  R and B run in the two
-// 16-bit lanes of one word, G in another: signed 16-bit adds, clamps with
-// v_pk_max_i16 / v_pk_min_i16 and a saturating v_pk_sub_u16.
+// ETC1 base-colour bytes of the block, from its average colour: each
+// channel of avgc moved by the luma correction (up: + corr, else - corr) with
+// saturation to 0..255 (goofy_tc.h:1431-1449), then cut to 5 bits in place,
+// q5(b) << 3 == subsat(b, 1) & 0xf8.  The correction is the signed
+// clamp(mid - Y(avgc), -127, 127), so a channel becomes
+// clamp(c + clamp(mid - Y, -127, 127), 0, 255); with the subtraction of one
+// folded in, clamp(c + clamp(mid - Y, -127, 127) - 1, 0, 254) & 0xf8.  R and
+// B run in the two 16-bit lanes of one word, G in another: one signed 16-bit
+// add and a v_pk_max_i16 / v_pk_min_i16 clamp per word.
 typedef short i16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t etc1_base_bytes(uint32_t avgc, uint32_t mid)
 {
-    const uint32_t ya = luma_b0(avgc);
-    const bool up = ya <= mid;
-    uint32_t corr = up ? mid - ya : ya - mid;
-    corr = corr < 127u ? corr : 127u;
-    const uint32_t sd = up ? corr : 0u - corr;   // low 16 bits: the signed correction
-    const uint32_t sd2 = perm(0u, sd, 0x01000100u);  // in both lanes
-    const i16x2 zero = {0, 0}, top = {255, 255};
-    const u16x2 one = {1, 1};
+    const int32_t ya = (int32_t)luma_b0(avgc);
+    const int32_t t = (int32_t)mid - ya - 1;
+    const int32_t sd = t < -128 ? -128 : (t > 126 ? 126 : t);  // the correction minus one
+    const uint32_t sd2 = perm(0u, (uint32_t)sd, 0x01000100u);   // in both lanes
+    const i16x2 zero = {0, 0}, top = {254, 254};
     auto base = [&](uint32_t lanes) {  // lanes: two channels in 16-bit lanes, 0..255 each
         const i16x2 v = __builtin_bit_cast(i16x2, lanes) + __builtin_bit_cast(i16x2, sd2);
         const i16x2 c = __builtin_elementwise_min(__builtin_elementwise_max(v, zero), top);
-        return __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, c), one)) &
-               0x00F800F8u;
+        return __builtin_bit_cast(uint32_t, c) & 0x00F800F8u;
     };
     return base(avgc & 0x00FF00FFu) | ((base((avgc >> 8) & 0x00FF00FFu) & 0xFFu) << 8);
 }
@@ -332,12 +333,12 @@ __device__ __forceinline__ uint2 encode_etc1(const uint32_t (&p)[16], const Bloc
     const uint32_t hi = s.mid + s.qt - 1u < 255u ? s.mid + s.qt - 1u : 255u;
     const uint32_t nmid4 = rep4(~s.mid);
     const uint32_t nlo4 = rep4(~lo);
-    const uint32_t hi4 = rep4(hi);
+    const uint32_t nhi4 = rep4(~hi);
 
     uint32_t gez[4], near[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-        compare4(luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]), nmid4, nlo4, hi4, &gez[r], &near[r]);
+        compare4(luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]), nmid4, nlo4, nhi4, &gez[r], &near[r]);
     const uint32_t w1 = etc1_selector_word(gez, near);
 
     // Average colour: rows first, then columns (goofy_tc.h:1402-1414).
@@ -374,11 +375,11 @@ __device__ __forceinline__ BlockClass classify(const uint32_t (&p)[16], const Bl
     const uint32_t hi = c.s.mid + c.s.qt - 1u < 255u ? c.s.mid + c.s.qt - 1u : 255u;  // y <= hi
     const uint32_t nmid4 = rep4(~c.s.mid);
     const uint32_t nlo4 = rep4(~lo);
-    const uint32_t hi4 = rep4(hi);
+    const uint32_t nhi4 = rep4(~hi);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const uint32_t y4 = luma4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]);
-        compare4(y4, nmid4, nlo4, hi4, &c.gez[r], &c.near[r]);
+        compare4(y4, nmid4, nlo4, nhi4, &c.gez[r], &c.near[r]);
     }
     return c;
 }
