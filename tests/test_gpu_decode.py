@@ -173,6 +173,47 @@ def test_decode_and_quality_address_paths(gpu, oracle, w, h, pitch):
         dst.free()
 
 
+def integer_sums(src, dec):
+    """The round-trip sums in integers: squared error of bytes 0..3, and
+    10^4 x the harness's luma error sum (its weights 0.21 / 0.72 / 0.07 on
+    bytes 2 / 1 / 0, as 21 / 72 / 7)."""
+    a = src.reshape(-1, 4)
+    b = dec.reshape(-1, 4)
+    d = [a[:, c].astype(np.int32) - b[:, c].astype(np.int32) for c in range(4)]
+    sse = [int((x.astype(np.int64) ** 2).sum()) for x in d]
+    y = 21 * d[2] + 72 * d[1] + 7 * d[0]
+    return sse, int((y.astype(np.int64) ** 2).sum())
+
+
+@pytest.mark.parametrize("fmt", [W.DXT1, W.ETC1, W.DXT5, W.ETC2])
+@pytest.mark.parametrize("w,h", [(2600, 2600), (3588, 3512)])
+def test_quality_waves_of_several_tiles(gpu, oracle, fmt, w, h):
+    """Images large enough that each wave of the round-trip kernel compares
+    two or three tiles of 64 blocks (its grid is at most 1536 workgroups of
+    four waves: a stride of 6144 tiles), with a partial last tile.  In 2600^2
+    (422,500 blocks, 6602 tiles) that tile is a wave's second; in 3588x3512
+    (787,566 blocks, 12,306 tiles) a wave's third.  BC1 / BC3 run these waves
+    with two tiles in registers.  Sums against the oracle's decode of the same
+    blocks, in integers."""
+    from goofy_amd import hip
+    kind = W.WAVES_ALPHA if fmt in (W.DXT5, W.ETC2) else W.WAVES
+    img = gpu.synth_host(kind, 31 + fmt, 0, 16, w, h, w * 4)
+    src = hip.DeviceBuffer.from_array(img)
+    blocks = hip.DeviceBuffer((w // 4) * (h // 4) * gpu.block_bytes(fmt))
+    try:
+        print(f"quality waves of several tiles {w}x{h} fmt={fmt}", flush=True)
+        gpu.compress_device(fmt, blocks, src, w, h, w * 4)
+        q = gpu.quality_device(fmt, blocks, src, w, h, w * 4)
+        dec = oracle.decode_image(fmt, blocks.download(), w, h)
+    finally:
+        src.free()
+        blocks.free()
+    sse, y100 = integer_sums(img, dec)
+    assert q["pixels"] == w * h
+    assert q["sse_byte"] == sse
+    assert q["sse_luma"] == pytest.approx(y100 / 1e4, rel=1e-12)
+
+
 @pytest.mark.parametrize("fmt", [W.DXT5, W.ETC2])
 def test_quality_alpha_formats_match_harness(gpu, harness, fmt):
     """Round trip of the alpha image through the 16-byte formats: the GPU
