@@ -239,13 +239,13 @@ __device__ __forceinline__ void sqerr_tile(const QTile<FMT>& t, bool valid, ErrS
 }
 
 // A wave's tiles first, first + step, ... in a grid-stride loop.
-// PIPELINE (BC1, BC3): two register sets in turn, the next tile's loads
-// issued before the current tile is compared; every path into a comparison
-// has the same loads outstanding after it, so its wait leaves the other
-// set's loads in flight (65 / 73 VGPRs, within the grid's 6 waves per SIMD;
-// quality calls 2-2.5% faster).  Otherwise one tile at a time (59 / 61
-// VGPRs for the ETC formats, whose two-set form needed 78-85 and waited for
-// both sets at every decode).
+// PIPELINE (BC1, ETC1, BC3): two register sets in turn, the next tile's
+// loads issued before the current tile is compared; every path into a
+// comparison has the same loads outstanding after it, so its wait leaves the
+// other set's loads in flight (65 / 78 / 73 VGPRs, within the grid's 6 waves
+// per SIMD; quality calls 2-3% faster).  ETC2 RGBA8 compares one tile at a
+// time (60 VGPRs): its two-set form needs 85, 5 waves per SIMD, and its
+// quality call took 8% longer.
 template <int FMT, bool NEAR, bool PIPELINE>
 __device__ __forceinline__ void sqerr_loop(const uint8_t* __restrict__ blocks, const uint8_t* __restrict__ src,
                                            uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void k_sqerr(const uint8_t* __restrict__ block
                                                uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
                                                QualitySums* __restrict__ part)
 {
-    constexpr bool kPipeline = FMT == 0 || FMT == 2;
+    constexpr bool kPipeline = FMT != 3;
     ErrSums acc = {};
     if (tile_near(bw, 64u, pitch)) sqerr_loop<FMT, true, kPipeline>(blocks, src, pitch, bw, nblocks, magic, acc);
     else sqerr_loop<FMT, false, kPipeline>(blocks, src, pitch, bw, nblocks, magic, acc);
