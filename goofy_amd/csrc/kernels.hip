@@ -161,12 +161,14 @@ __device__ __forceinline__ void encode_tile(const uint8_t* __restrict__ src, uin
 // through one contiguous region instead of every eighth workgroup's tiles.
 // A bijection on [0, gridDim.x): r = n % 8 XCDs hold q + 1 workgroups, the
 // rest q.  All operands are wave-uniform (scalar unit).
-// blockDim.x is a 16-bit field of the dispatch packet, which this target
-// reads with a vector load at the head of every wave.  Passing the wave count
-// as a kernel argument instead measured 2% slower on the DXT1 / ETC1S
-// kernels at 8192^2 (HBM-bound) and no different on the alpha ones, so the
-// read stays.
-__device__ __forceinline__ uint32_t wave_index(uint32_t order)
+// wpg = waves per workgroup.  The batch kernels take it as an argument; the
+// others read blockDim.x, a 16-bit field of the dispatch packet, which this
+// target reads with a vector load at the head of every wave.  Measured both
+// ways: on the DXT1 / ETC1S kernels at 8192^2 (HBM-bound) the read is 2%
+// faster than the argument, and no different on the alpha ones; on the
+// 4096-texture batch, whose waves first search the work-list, the argument is
+// 2-3% faster.
+__device__ __forceinline__ uint32_t wave_index(uint32_t order, uint32_t wpg)
 {
     uint32_t b = blockIdx.x;
     if (order) {
@@ -174,8 +176,10 @@ __device__ __forceinline__ uint32_t wave_index(uint32_t order)
         const uint32_t x = b % kXcdCount, k = b / kXcdCount;
         b = x < r ? x * (q + 1u) + k : r * (q + 1u) + (x - r) * q + k;
     }
-    return __builtin_amdgcn_readfirstlane(b * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    return __builtin_amdgcn_readfirstlane(b * wpg + (threadIdx.x >> 6));
 }
+
+__device__ __forceinline__ uint32_t wave_index(uint32_t order) { return wave_index(order, blockDim.x >> 6); }
 
 // One image; wave w encodes tile w.
 template <int FMT, int BPL, bool NT>
@@ -241,9 +245,9 @@ __global__ __launch_bounds__(512) void k_encode_dual_alpha(const uint8_t* __rest
 // of DXT1 / ETC1S textures only run the instance without those branches.
 template <int BPL, bool NT, bool ALPHA, class Descs, class Begins>
 __device__ __forceinline__ void batch_body(const Descs& descs, const Begins& tile_begin, uint32_t ntex,
-                                           uint32_t ntiles, uint32_t tpw, uint32_t order)
+                                           uint32_t ntiles, uint32_t tpw, uint32_t order, uint32_t wpg)
 {
-    const uint32_t wave = wave_index(order);
+    const uint32_t wave = wave_index(order, wpg);
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t t = wave * tpw;
     const uint32_t t_end = min(t + tpw, ntiles);
@@ -282,17 +286,17 @@ __device__ __forceinline__ void batch_body(const Descs& descs, const Begins& til
 template <int BPL, bool NT, bool ALPHA>
 __global__ __launch_bounds__(512) void k_encode_batch(const TexDesc* __restrict__ descs,
                                                       const uint32_t* __restrict__ tile_begin, uint32_t ntex,
-                                                      uint32_t ntiles, uint32_t tpw, uint32_t order)
+                                                      uint32_t ntiles, uint32_t tpw, uint32_t order, uint32_t wpg)
 {
-    batch_body<BPL, NT, ALPHA>(descs, tile_begin, ntex, ntiles, tpw, order);
+    batch_body<BPL, NT, ALPHA>(descs, tile_begin, ntex, ntiles, tpw, order, wpg);
 }
 
 // The same with the work-list in the kernel arguments (small groups).
 template <int BPL, bool NT, bool ALPHA>
 __global__ __launch_bounds__(512) void k_encode_batch_inline(const InlineWorklist wl, uint32_t ntex, uint32_t ntiles,
-                                                             uint32_t tpw, uint32_t order)
+                                                             uint32_t tpw, uint32_t order, uint32_t wpg)
 {
-    batch_body<BPL, NT, ALPHA>(wl.d, wl.tile_begin, ntex, ntiles, tpw, order);
+    batch_body<BPL, NT, ALPHA>(wl.d, wl.tile_begin, ntex, ntiles, tpw, order, wpg);
 }
 
 __global__ __launch_bounds__(256) void k_synth(goofy_synth::Params P, uint8_t* __restrict__ dst, uint32_t width,
@@ -396,8 +400,11 @@ template <int BPL, bool ALPHA>
 static hipError_t batch_go(bool nt, dim3 grid, dim3 block, const TexDesc* d, const uint32_t* tb, uint32_t ntex,
                            uint32_t ntiles, uint32_t tpw, uint32_t order, hipStream_t s)
 {
-    if (nt) hipLaunchKernelGGL((k_encode_batch<BPL, true, ALPHA>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order);
-    else hipLaunchKernelGGL((k_encode_batch<BPL, false, ALPHA>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order);
+    const uint32_t wpg = block.x / 64u;
+    if (nt)
+        hipLaunchKernelGGL((k_encode_batch<BPL, true, ALPHA>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order, wpg);
+    else
+        hipLaunchKernelGGL((k_encode_batch<BPL, false, ALPHA>), grid, block, 0, s, d, tb, ntex, ntiles, tpw, order, wpg);
     return hipGetLastError();
 }
 
@@ -431,10 +438,13 @@ template <int BPL, bool ALPHA>
 static hipError_t batch_inline_go(bool nt, dim3 grid, dim3 block, const InlineWorklist& wl, uint32_t ntex,
                                   uint32_t ntiles, uint32_t tpw, uint32_t order, hipStream_t s)
 {
+    const uint32_t wpg = block.x / 64u;
     if (nt)
-        hipLaunchKernelGGL((k_encode_batch_inline<BPL, true, ALPHA>), grid, block, 0, s, wl, ntex, ntiles, tpw, order);
+        hipLaunchKernelGGL((k_encode_batch_inline<BPL, true, ALPHA>), grid, block, 0, s, wl, ntex, ntiles, tpw, order,
+                           wpg);
     else
-        hipLaunchKernelGGL((k_encode_batch_inline<BPL, false, ALPHA>), grid, block, 0, s, wl, ntex, ntiles, tpw, order);
+        hipLaunchKernelGGL((k_encode_batch_inline<BPL, false, ALPHA>), grid, block, 0, s, wl, ntex, ntiles, tpw, order,
+                           wpg);
     return hipGetLastError();
 }
 
