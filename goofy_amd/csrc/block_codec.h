@@ -7,8 +7,9 @@
 // most of their instructions on transposes (transposeAs4x4 :263-276,
 // deinterleaveRGB :278-299, transposeAs4x4x4 :327-358); here there is no
 // transpose at all: v_perm_b32 gathers bytes, v_lerp_u8 is the per-byte
-// rounding average (_mm_avg_epu8), packed 16-bit min/max give the bounding
-// box, and per-byte comparisons come out of the same rounding average.
+// rounding average (_mm_avg_epu8), packed 16-bit min/max (three-input ones
+// where the lanes hold bytes) give the bounding box, and per-byte comparisons
+// come out of the same rounding average.
 //
 // Integer semantics (all values u8, avg(a,b) = (a+b+1)>>1):
 //   Y(r,g,b) = avg(avg(r,b),g)                       goofy_tc.h:1166,1205
