@@ -360,6 +360,37 @@ def test_quality_sums_past_one_launch_closed_form(gpu):
                                                                   abs=1e-9)
 
 
+@pytest.mark.parametrize("fmt", [W.ETC1, W.ETC2])
+def test_quality_sums_past_one_launch_etc_closed_form(gpu, oracle, fmt):
+    """The same 819,200,000 blocks for the ETC round trips, which add two
+    pixels per v_dot2_i32_i16 into signed 32-bit lane sums.  All-zero ETC1
+    blocks decode to (2, 2, 2, 255) and ETC2 RGBA8 ones to (2, 2, 2, 0) (the
+    oracle's decoder, asserted here), so a white source errs by 253 in bytes
+    0..2 and, for ETC2 RGBA8, by 255 in alpha: a lane's alpha sum reaches
+    2048 x 16 x 65025 = 2,130,739,200, just under 2^31."""
+    from goofy_amd import hip
+    bb = gpu.block_bytes(fmt)
+    px = oracle.decode_image(fmt, np.zeros(bb, np.uint8), 4, 4).reshape(16, 4)
+    assert (px == px[0]).all()
+    want_px = [2, 2, 2, 255] if fmt == W.ETC1 else [2, 2, 2, 0]
+    assert px[0].tolist() == want_px
+    err = [255 - v for v in want_px]
+    w, h = 4096, 3_200_000
+    pixels = w * h
+    print(f"ETC quality past one launch: fmt={fmt} {w}x{h}, {pixels // 16} blocks", flush=True)
+    src = hip.DeviceBuffer(w * 4 * h, fill=0xFF)
+    blocks = hip.DeviceBuffer((w // 4) * (h // 4) * bb, fill=0)
+    try:
+        q = gpu.quality_device(fmt, blocks, src, w, h, w * 4)
+    finally:
+        src.free()
+        blocks.free()
+    assert q["pixels"] == pixels
+    assert q["sse_byte"] == [e * e * pixels for e in err]
+    y = 21 * err[2] + 72 * err[1] + 7 * err[0]
+    assert q["sse_luma"] == pytest.approx(y * y * pixels / 1e4, rel=1e-15)
+
+
 def test_quality_images_closed_form(gpu):
     from goofy_amd import hip
     w, h = 64, 8
