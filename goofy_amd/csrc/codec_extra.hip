@@ -234,7 +234,6 @@ __device__ __forceinline__ void block_sums_store(unsigned long long (&e)[4], uns
 // each lane's block and its four source rows.  Lanes past the last block
 // load the last block again and skip its sums, so every load is issued
 // unconditionally.
-template <int FMT>
 struct QTile {
     uint4 b;
     u32x4 v[4];
@@ -243,7 +242,7 @@ struct QTile {
 template <int FMT, bool NEAR>
 __device__ __forceinline__ void load_qtile(const uint8_t* __restrict__ blocks, const uint8_t* __restrict__ src,
                                            uint64_t pitch, uint32_t bw, uint32_t nblocks, uint64_t magic,
-                                           uint32_t first, uint32_t lane, QTile<FMT>& t)
+                                           uint32_t first, uint32_t lane, QTile& t)
 {
     const uint32_t k = min(lane, nblocks - 1u - first), i = first + k;
     t.b = load_block_fmt<FMT>(blocks, i);
@@ -265,7 +264,7 @@ __device__ __forceinline__ void load_qtile(const uint8_t* __restrict__ blocks, c
 // Decode the lane's block and add its errors against the source rows, each
 // decoded row compared as it comes (4 decoded pixels live).
 template <int FMT>
-__device__ __forceinline__ void sqerr_tile(const QTile<FMT>& t, bool valid, ErrSums& acc)
+__device__ __forceinline__ void sqerr_tile(const QTile& t, bool valid, ErrSums& acc)
 {
     if (valid)
         decode_block_rows<FMT>(t.b, [&](int y, const uint32_t (&row)[4]) {
@@ -277,7 +276,7 @@ __device__ __forceinline__ void sqerr_tile(const QTile<FMT>& t, bool valid, ErrS
 // PIPELINE (BC1, ETC1, BC3): two register sets in turn, the next tile's
 // loads issued before the current tile is compared; every path into a
 // comparison has the same loads outstanding after it, so its wait leaves the
-// other set's loads in flight (65 / 78 / 73 VGPRs, within the grid's 6 waves
+// other set's loads in flight (65 / 75 / 73 VGPRs, within the grid's 6 waves
 // per SIMD; quality calls 2-3% faster).  ETC2 RGBA8 compares one tile at a
 // time (60 VGPRs): its two-set form needs 85, 5 waves per SIMD, and its
 // quality call took 8% longer.
@@ -290,7 +289,7 @@ __device__ __forceinline__ void sqerr_loop(const uint8_t* __restrict__ blocks, c
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u));
     if (t >= nblocks) return;
-    QTile<FMT> a, b;
+    QTile a, b;
     if constexpr (PIPELINE) {
         uint32_t n = (nblocks - 1u - t) / step + 1u;  // tiles of this wave; all but the last are whole
         load_qtile<FMT, NEAR>(blocks, src, pitch, bw, nblocks, magic, t, lane, a);
